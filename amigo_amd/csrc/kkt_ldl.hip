@@ -1,0 +1,1582 @@
+// MI355X (gfx950) supernodal LDL^T for interior-point KKT systems.
+//
+// Design (one GPU, HBM-resident):
+//  * Host: ordering + symbolic analysis (symbolic.cpp), then a device plan:
+//    every supernode ("front") owns a dense column-major slot in one HBM
+//    arena that it keeps for the whole factor/solve lifetime.  Nothing is
+//    ever copied onto a stack: the factor L of a node is the leading
+//    npiv columns of its front, and its contribution (Schur) block is the
+//    trailing block of the same front, read in place by the parent.
+//  * Fronts are processed level by level (leaves first).  Per level:
+//      k_assemble  -- pull-based extend-add: each front entry is written by
+//                     exactly one thread that sums the children's
+//                     contributions in child order (deterministic, no
+//                     atomics), then the node's original K entries and the
+//                     barrier diagonal are added.
+//      k_factor    -- one workgroup per front: the reference's blocked
+//                     threshold 1x1/2x2 pivoting with delayed pivots
+//                     (include/sparse_ldl.h:1316-1553), or Bunch-Kaufman for
+//                     roots (the reference calls LAPACK dsytrf there,
+//                     :1226-1248), with the W panel in LDS and an in-LDS
+//                     trailing update for small fronts.
+//      k_panel + k_update (large fronts) -- the same panel on one workgroup,
+//                     the rank-64 trailing update tiled over many
+//                     workgroups with FP64 MFMA.
+//  * Delayed pivots make front sizes data dependent: each front slot has
+//    slack; a front that overflows marks itself and its ancestors failed,
+//    the host enlarges the slack from the observed counts and refactors.
+//  * Solve: multifrontal forward pass (front vectors pulled from the
+//    children's update vectors, same maps as the assembly) and a backward
+//    pass that reads ancestors' solution entries; level scheduled.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kkt_ldl.h"
+#include "symbolic.hpp"
+
+namespace kktldl {
+
+static thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& s) : std::runtime_error(s), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                        \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess)                                                     \
+      throw Error(-2, std::string("HIP error ") + hipGetErrorString(_e) +     \
+                          " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+constexpr int kNB = 64;        // panel width bound (reference block_size)
+constexpr int kTPB = 256;      // threads per workgroup
+constexpr int kWave = 64;
+constexpr int kAsmCols = 8;    // columns per assembly workgroup
+
+// --------------------------------------------------------------------------
+// Device-side plan (pointers into device memory).
+// --------------------------------------------------------------------------
+struct DevPlan {
+  // static per node
+  const int* ns;       // own (fully summed nominal) columns
+  const int* cs;       // nominal contribution rows
+  const int* cap;      // front slot dimension
+  const int* vptr;     // node_ptr
+  const int* cptr;     // contrib_ptr
+  const int* chptr;    // children list pointer [N+1]
+  const int* ch;       // children
+  const long long* invoff;  // per node (as child): offset of its inverse map
+  const long long* foff;    // front offset (elements)
+  const long long* ioff;    // vars/pre offset
+  const long long* woff;    // panel scratch offset (elements)
+  const long long* uoff;    // solve update-vector offset (elements, per rhs)
+  const int* mode;     // 0 threshold, 1 Bunch-Kaufman root, 2 Cholesky
+  const int* node_to_var;
+  const int* contrib_rows;
+  const int* invnom;   // for each child: parent's nominal position -> child contrib index or -1
+  const int* am_ptr;   // per own column (global index g in node_to_var order)
+  const int* am_src;   // CSR index into vals
+  const int* am_row;   // nominal row in the front
+  // dynamic per node
+  int* din;            // delayed pivots received
+  int* m;              // actual front dimension
+  int* npiv;
+  int* ndel;
+  int* doff;           // offset of this node's delays inside the parent's section
+  int* stat;           // 0 ok, 1 overflow / failed
+  int* need;           // front dimension required when overflowing
+  int* cnt;            // [3N]: npos, nneg, n2x2
+  // arenas
+  double* F;
+  int* vars;
+  int* pre;
+  int* ptype;          // per front position: 1 = 1x1, 2/3 = first/second of 2x2
+  double* W;
+  double* U;
+  // numeric input
+  const double* vals;
+  const double* diag;
+  double ustab, ptol;
+};
+
+__device__ inline int unpiv(int v) { return v >= 0 ? v : -v - 1; }
+
+// Row of child c's front (post-pivot) that holds the parent's pre-pivot
+// position i, or -1.  Children contribute their delayed pivots (rows
+// npiv_c .. fs_c-1) and their nominal contribution rows (fs_c + q).
+struct ChildView {
+  const double* F;
+  int m, fs, npiv, ndel, doff;
+  long long inv;
+};
+
+__device__ inline ChildView load_child(const DevPlan& P, int c) {
+  ChildView v;
+  v.F = P.F + P.foff[c];
+  v.m = P.m[c];
+  v.fs = P.ns[c] + P.din[c];
+  v.npiv = P.npiv[c];
+  v.ndel = P.ndel[c];
+  v.doff = P.doff[c];
+  v.inv = P.invoff[c];
+  return v;
+}
+
+__device__ inline int child_row(const DevPlan& P, const ChildView& cv, int i,
+                                int ns, int din) {
+  if (i < ns) {
+    int q = P.invnom[cv.inv + i];
+    return q >= 0 ? cv.fs + q : -1;
+  }
+  if (i < ns + din) {
+    int d = i - ns - cv.doff;
+    return (d >= 0 && d < cv.ndel) ? cv.npiv + d : -1;
+  }
+  int q = P.invnom[cv.inv + (i - din)];
+  return q >= 0 ? cv.fs + q : -1;
+}
+
+// --------------------------------------------------------------------------
+// Per-level preparation: delayed pivots received from the children, their
+// offsets inside this front, the actual front dimension, overflow check.
+// --------------------------------------------------------------------------
+__global__ void k_prepare(DevPlan P, const int* nodes, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int p = nodes[t];
+  int din = 0, fail = 0;
+  for (int q = P.chptr[p]; q < P.chptr[p + 1]; q++) {
+    const int c = P.ch[q];
+    if (P.stat[c]) {
+      fail = 1;
+      continue;
+    }
+    P.doff[c] = din;
+    din += P.ndel[c];
+  }
+  const int m = P.ns[p] + din + P.cs[p];
+  P.din[p] = din;
+  P.m[p] = m;
+  if (fail || m > P.cap[p]) {
+    P.stat[p] = 1;
+    P.need[p] = fail ? -1 : m;
+  } else {
+    P.stat[p] = 0;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Assembly: one workgroup per (node, block of kAsmCols columns).  Each
+// front entry (i >= j) is produced by one thread as the sum of the
+// children's contribution entries that land on it, in child order.
+// --------------------------------------------------------------------------
+constexpr int kChildCache = 64;
+
+__global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
+                                                   const int* wchunk) {
+  const int p = wnode[blockIdx.x];
+  const int chunk = wchunk[blockIdx.x];
+  const int tid = threadIdx.x;
+  if (P.stat[p]) return;
+  const int c0 = P.chptr[p], nch = P.chptr[p + 1] - c0;
+  const int ns = P.ns[p], cs = P.cs[p], din = P.din[p], m = P.m[p];
+  const int j0 = chunk * kAsmCols;
+  if (j0 >= m) return;
+  const int j1 = min(m, j0 + kAsmCols);
+  __shared__ ChildView s_cv[kChildCache];
+  __shared__ int s_list[kChildCache];
+  __shared__ int s_rj[kChildCache];
+  __shared__ int s_nl;
+  const int ncache = min(nch, kChildCache);
+  for (int t = tid; t < ncache; t += kTPB) s_cv[t] = load_child(P, P.ch[c0 + t]);
+  int* vars = P.vars + P.ioff[p];
+  int* pre = P.pre + P.ioff[p];
+  if (chunk == 0) {
+    const int vp = P.vptr[p], cp = P.cptr[p];
+    for (int i = tid; i < ns; i += kTPB) vars[i] = P.node_to_var[vp + i];
+    for (int i = tid; i < cs; i += kTPB) vars[ns + din + i] = P.contrib_rows[cp + i];
+    for (int i = tid; i < m; i += kTPB) pre[i] = i;
+    for (int q = 0; q < nch; q++) {
+      const int c = P.ch[c0 + q];
+      const int nd = P.ndel[c], off = P.doff[c];
+      const int* cv = P.vars + P.ioff[c] + P.npiv[c];
+      for (int t = tid; t < nd; t += kTPB) vars[ns + off + t] = cv[t];
+    }
+  }
+  __syncthreads();
+  double* F = P.F + P.foff[p];
+  const bool cached = nch <= kChildCache;
+  for (int j = j0; j < j1; j++) {
+    double* Fj = F + (size_t)j * m;
+    if (cached) {
+      // children that own position j, compacted in child order (wave 0)
+      if (tid < kWave) {
+        const int rj = tid < nch ? child_row(P, s_cv[tid], j, ns, din) : -1;
+        const unsigned long long mask = __ballot(rj >= 0);
+        const int pos = __popcll(mask & ((1ull << tid) - 1ull));
+        if (rj >= 0) {
+          s_list[pos] = tid;
+          s_rj[pos] = rj;
+        }
+        if (tid == 0) s_nl = __popcll(mask);
+      }
+      __syncthreads();
+      const int nl = s_nl;
+      for (int i = j + tid; i < m; i += kTPB) {
+        double v = 0.0;
+        for (int t = 0; t < nl; t++) {
+          const ChildView& cv = s_cv[s_list[t]];
+          const int ri = child_row(P, cv, i, ns, din);
+          if (ri >= 0) {
+            const int rj = s_rj[t];
+            v += cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
+          }
+        }
+        Fj[i] = v;
+      }
+      __syncthreads();
+    } else {
+      for (int i = j + tid; i < m; i += kTPB) {
+        double v = 0.0;
+        for (int q = 0; q < nch; q++) {
+          ChildView cv = load_child(P, P.ch[c0 + q]);
+          const int rj = child_row(P, cv, j, ns, din);
+          if (rj < 0) continue;
+          const int ri = child_row(P, cv, i, ns, din);
+          if (ri >= 0) v += cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
+        }
+        Fj[i] = v;
+      }
+    }
+  }
+  __syncthreads();
+  // original entries of the own columns in [j0, j1): distinct positions
+  const int vp = P.vptr[p];
+  for (int j = j0; j < min(j1, ns); j++) {
+    const int g = vp + j;
+    double* Fj = F + (size_t)j * m;
+    for (int e = P.am_ptr[g] + tid; e < P.am_ptr[g + 1]; e += kTPB) {
+      const int u = P.am_row[e];
+      Fj[u < ns ? u : u + din] += P.vals[P.am_src[e]];
+    }
+  }
+  __syncthreads();
+  if (P.diag)
+    for (int j = j0 + tid; j < min(j1, ns); j += kTPB)
+      F[(size_t)j * m + j] += P.diag[P.node_to_var[vp + j]];
+}
+
+// --------------------------------------------------------------------------
+// Workgroup reductions.
+// --------------------------------------------------------------------------
+struct RedBuf {
+  double v[kTPB / kWave];
+  int i[kTPB / kWave];
+  double v2[kTPB / kWave];
+};
+
+// Block-wide (max |x|, first index) and second plain max; results broadcast.
+__device__ inline void block_argmax2(double& val, int& idx, double& val2,
+                                     RedBuf& rb) {
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    double ov = __shfl_xor(val, off);
+    int oi = __shfl_xor(idx, off);
+    double o2 = __shfl_xor(val2, off);
+    if (ov > val || (ov == val && oi < idx)) {
+      val = ov;
+      idx = oi;
+    }
+    val2 = fmax(val2, o2);
+  }
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  __syncthreads();
+  if (l == 0) {
+    rb.v[w] = val;
+    rb.i[w] = idx;
+    rb.v2[w] = val2;
+  }
+  __syncthreads();
+  val = rb.v[0];
+  idx = rb.i[0];
+  val2 = rb.v2[0];
+  for (int t = 1; t < kTPB / kWave; t++) {
+    if (rb.v[t] > val || (rb.v[t] == val && rb.i[t] < idx)) {
+      val = rb.v[t];
+      idx = rb.i[t];
+    }
+    val2 = fmax(val2, rb.v2[t]);
+  }
+}
+
+// --------------------------------------------------------------------------
+// Front factorization (one workgroup).  F: front (ld = m), W: panel
+// workspace (ld = ldw, kNB columns).
+// --------------------------------------------------------------------------
+struct FrontShared {
+  double wr[kNB];
+  double wr2[kNB];
+  RedBuf rb;
+  int npos, nneg, n2;
+};
+
+__device__ inline void sym_swap(double* F, int ld, int m, int* vars, int* pre,
+                                int a, int b) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    int t = vars[a];
+    vars[a] = vars[b];
+    vars[b] = t;
+    t = pre[a];
+    pre[a] = pre[b];
+    pre[b] = t;
+    double d = F[a + (size_t)a * ld];
+    F[a + (size_t)a * ld] = F[b + (size_t)b * ld];
+    F[b + (size_t)b * ld] = d;
+  }
+  for (int j = tid; j < a; j += kTPB) {
+    double t = F[a + (size_t)j * ld];
+    F[a + (size_t)j * ld] = F[b + (size_t)j * ld];
+    F[b + (size_t)j * ld] = t;
+  }
+  for (int j = a + 1 + tid; j < b; j += kTPB) {
+    double t = F[j + (size_t)a * ld];
+    F[j + (size_t)a * ld] = F[b + (size_t)j * ld];
+    F[b + (size_t)j * ld] = t;
+  }
+  for (int i = b + 1 + tid; i < m; i += kTPB) {
+    double t = F[i + (size_t)a * ld];
+    F[i + (size_t)a * ld] = F[i + (size_t)b * ld];
+    F[i + (size_t)b * ld] = t;
+  }
+}
+
+// One panel of at most nb-1 pivots starting at column k.  Returns with k,
+// nc advanced; W[k0:, 0:kw] and F[:, kstart:k] hold the panel for the
+// trailing update.  mode 0: threshold pivoting with delays; mode 1:
+// Bunch-Kaufman (every column is a candidate, nothing is delayed).
+__device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
+                             int* ptype, double* W, int ldw, int nb, int mode,
+                             double ustab,
+                             double ptol, int& k, int& nc, int& kw_out,
+                             FrontShared& sh) {
+  const int tid = threadIdx.x;
+  const double alpha = 0.6403882032022076;  // (1 + sqrt(17)) / 8
+  const int kstart = k;
+  int kw = 0;
+  while (kw < nb - 1 && k < nc) {
+    // W[k:m, kw] = F[k:m, k] - F[k:m, kstart:k] W[k, 0:kw]^T
+    for (int c = tid; c < kw; c += kTPB) sh.wr[c] = W[k + (size_t)c * ldw];
+    __syncthreads();
+    double lbest = -1.0, lrest = 0.0;
+    int lidx = 0x7fffffff;
+    for (int i = k + tid; i < m; i += kTPB) {
+      double v = F[i + (size_t)k * ld];
+      for (int c = 0; c < kw; c++) v -= F[i + (size_t)(kstart + c) * ld] * sh.wr[c];
+      W[i + (size_t)kw * ldw] = v;
+      double a = fabs(v);
+      if (i > k && i < nc) {
+        if (a > lbest) {
+          lbest = a;
+          lidx = i;
+        }
+      } else if (i >= nc) {
+        lrest = fmax(lrest, a);
+      }
+    }
+    block_argmax2(lbest, lidx, lrest, sh.rb);
+    const double awkk = fabs(W[k + (size_t)kw * ldw]);
+    int imax = 0;
+    double awik = 0.0;
+    if (k + 1 < nc) {
+      imax = lidx;
+      awik = W[imax + (size_t)kw * ldw];
+      awik = fabs(awik);
+    }
+    const double colmax = fmax(awik, lrest);
+    int step = 0, kpiv = -1;
+    bool zero_col = false;
+    bool need_row = false;
+    if (mode == 1) {
+      if (fmax(awkk, colmax) == 0.0 || isnan(awkk)) {
+        step = 1;
+        kpiv = k;
+        zero_col = true;
+      } else if (awkk >= alpha * colmax) {
+        step = 1;
+        kpiv = k;
+      } else {
+        need_row = true;
+      }
+    } else {
+      if (awkk >= ptol && awkk >= ustab * colmax) {
+        step = 1;
+        kpiv = k;
+      } else if (k + 1 < nc) {
+        need_row = true;
+      }
+    }
+    if (need_row) {
+      // W[k:m, kw+1] = column imax updated by the panel so far
+      for (int c = tid; c < kw; c += kTPB) sh.wr2[c] = W[imax + (size_t)c * ldw];
+      __syncthreads();
+      double* W1 = W + (size_t)(kw + 1) * ldw;
+      double lrow = 0.0;
+      for (int i = k + tid; i < m; i += kTPB) {
+        double v = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
+        for (int c = 0; c < kw; c++) v -= F[i + (size_t)(kstart + c) * ld] * sh.wr2[c];
+        W1[i] = v;
+        if (i != imax) lrow = fmax(lrow, fabs(v));
+      }
+      double dummy = -1.0;
+      int di = 0;
+      block_argmax2(dummy, di, lrow, sh.rb);
+      const double rowmax = lrow;
+      const double awii = fabs(W1[imax]);
+      if (mode == 1) {
+        if (awkk >= alpha * colmax * (colmax / rowmax)) {
+          step = 1;
+          kpiv = k;
+        } else if (awii >= alpha * rowmax) {
+          step = 1;
+          kpiv = imax;
+        } else {
+          step = 2;
+          kpiv = imax;
+        }
+      } else {
+        if (awii >= ptol && awii >= ustab * rowmax) {
+          step = 1;
+          kpiv = imax;
+        } else {
+          double wkk = W[k + (size_t)kw * ldw], wik = W[imax + (size_t)kw * ldw];
+          double det = fabs(wkk * W1[imax] - wik * wik);
+          if (det >= ptol) {
+            double delta = det / fmax(awii + awik, awkk + awik);
+            if (delta >= ustab * colmax && delta >= ustab * rowmax) {
+              step = 2;
+              kpiv = imax;
+            }
+          }
+        }
+      }
+      if (step == 1 && kpiv == imax) {
+        for (int i = k + tid; i < m; i += kTPB) W[i + (size_t)kw * ldw] = W1[i];
+      }
+      __syncthreads();
+    }
+    if (step == 0) {  // delay candidate k (threshold mode only)
+      if (k < nc - 1) {
+        sym_swap(F, ld, m, vars, pre, k, nc - 1);
+        for (int c = tid; c < kw; c += kTPB) {
+          double t = W[k + (size_t)c * ldw];
+          W[k + (size_t)c * ldw] = W[nc - 1 + (size_t)c * ldw];
+          W[nc - 1 + (size_t)c * ldw] = t;
+        }
+      }
+      nc -= 1;
+      __syncthreads();
+      continue;
+    }
+    const int ksrc = k + step - 1;
+    if (kpiv != ksrc) {
+      sym_swap(F, ld, m, vars, pre, ksrc, kpiv);
+      for (int c = tid; c < kw + step; c += kTPB) {
+        double t = W[ksrc + (size_t)c * ldw];
+        W[ksrc + (size_t)c * ldw] = W[kpiv + (size_t)c * ldw];
+        W[kpiv + (size_t)c * ldw] = t;
+      }
+      __syncthreads();
+    }
+    if (step == 1) {
+      const double d = W[k + (size_t)kw * ldw];
+      const double dinv = (zero_col || d == 0.0) ? 0.0 : 1.0 / d;
+      for (int i = k + tid; i < m; i += kTPB) {
+        double v = W[i + (size_t)kw * ldw];
+        F[i + (size_t)k * ld] = (i == k || zero_col) ? v : v * dinv;
+      }
+      if (tid == 0) {
+        ptype[k] = 1;
+        if (d >= 0.0) sh.npos++;
+        else sh.nneg++;
+      }
+    } else {
+      const double d11 = W[k + (size_t)kw * ldw];
+      const double d21 = W[k + 1 + (size_t)kw * ldw];
+      const double d22 = W[k + 1 + (size_t)(kw + 1) * ldw];
+      const double det = d11 * d22 - d21 * d21;
+      const double detinv = 1.0 / det;
+      const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
+      for (int i = k + 2 + tid; i < m; i += kTPB) {
+        double a = W[i + (size_t)kw * ldw], b = W[i + (size_t)(kw + 1) * ldw];
+        F[i + (size_t)k * ld] = a * i00 + b * i01;
+        F[i + (size_t)(k + 1) * ld] = a * i01 + b * i11;
+      }
+      if (tid == 0) {
+        F[k + (size_t)k * ld] = d11;
+        F[k + 1 + (size_t)k * ld] = d21;
+        F[k + 1 + (size_t)(k + 1) * ld] = d22;
+        vars[k] = -(vars[k] + 1);
+        vars[k + 1] = -(vars[k + 1] + 1);
+        ptype[k] = 2;
+        ptype[k + 1] = 3;
+        if (det >= 0.0) {
+          if (d11 >= 0.0) sh.npos += 2;
+          else sh.nneg += 2;
+        } else {
+          sh.npos++;
+          sh.nneg++;
+        }
+        sh.n2++;
+      }
+    }
+    k += step;
+    kw += step;
+    __syncthreads();
+  }
+  kw_out = kw;
+}
+
+// F[k:m, k:m] (lower) -= F[k:m, kstart:kstart+kw] * W[k:m, 0:kw]^T, one
+// workgroup, 4x4 register tiles.
+__device__ void trailing_update_wg(double* F, int ld, int m, const double* W,
+                                   int ldw, int kstart, int k, int kw) {
+  const int nt = m - k;
+  if (nt <= 0 || kw == 0) return;
+  const int nb4 = (nt + 3) / 4;
+  const int ntiles = nb4 * (nb4 + 1) / 2;
+  for (int t = threadIdx.x; t < ntiles; t += kTPB) {
+    // map t -> (bi >= bj)
+    int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while (bi * (bi + 1) / 2 > t) bi--;
+    while ((bi + 1) * (bi + 2) / 2 <= t) bi++;
+    const int bj = t - bi * (bi + 1) / 2;
+    const int i0 = k + 4 * bi, j0 = k + 4 * bj;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+    for (int c = 0; c < kw; c++) {
+      const double* Lc = F + (size_t)(kstart + c) * ld;
+      const double* Wc = W + (size_t)c * ldw;
+      double l[4], w[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        l[a] = (i0 + a < m) ? Lc[i0 + a] : 0.0;
+        w[a] = (j0 + a < m) ? Wc[j0 + a] : 0.0;
+      }
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] += l[a] * w[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        int i = i0 + a, j = j0 + b;
+        if (i < m && j < m && i >= j) F[i + (size_t)j * ld] -= acc[a][b];
+      }
+  }
+  __syncthreads();
+}
+
+// Cholesky mode (include/sparse_ldl.h:2010-2044): right-looking, one
+// workgroup; used by the reference's tests only.
+__device__ void cholesky_front(double* F, int ld, int m, int fs) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < fs; k++) {
+    const double d = sqrt(F[k + (size_t)k * ld]);
+    __syncthreads();
+    for (int i = k + 1 + tid; i < m; i += kTPB) F[i + (size_t)k * ld] /= d;
+    if (tid == 0) F[k + (size_t)k * ld] = d;
+    __syncthreads();
+    const int nt = m - k - 1;
+    for (int e = tid; e < nt * nt; e += kTPB) {
+      int jj = e / nt, ii = e % nt;
+      if (ii < jj) continue;
+      int i = k + 1 + ii, j = k + 1 + jj;
+      F[i + (size_t)j * ld] -= F[i + (size_t)k * ld] * F[j + (size_t)k * ld];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
+  const int p = nodes[blockIdx.x];
+  if (P.stat[p]) return;
+  __shared__ FrontShared sh;
+  const int tid = threadIdx.x;
+  const int m = P.m[p];
+  const int fs = P.ns[p] + P.din[p];
+  const int mode = P.mode[p];
+  double* F = P.F + P.foff[p];
+  int* vars = P.vars + P.ioff[p];
+  int* pre = P.pre + P.ioff[p];
+  int* ptype = P.ptype + P.ioff[p];
+  double* W = P.W + P.woff[p];
+  const int ldw = P.cap[p];
+  if (tid == 0) {
+    sh.npos = 0;
+    sh.nneg = 0;
+    sh.n2 = 0;
+  }
+  __syncthreads();
+  if (mode == 2) {
+    cholesky_front(F, m, m, fs);
+    for (int j = tid; j < fs; j += kTPB) ptype[j] = 1;
+    if (tid == 0) {
+      P.npiv[p] = fs;
+      P.ndel[p] = 0;
+      P.cnt[3 * p] = 0;
+      P.cnt[3 * p + 1] = 0;
+      P.cnt[3 * p + 2] = 0;
+    }
+    return;
+  }
+  // panel width as the reference (min(block_size, front_size)); at least 2
+  // so that a 1 x 1 root front still takes its single pivot
+  const int nb = max(2, min(kNB, m));
+  int k = 0, nc = (mode == 1) ? m : fs;
+  while (k < nc) {
+    const int kstart = k;
+    int kw = 0;
+    panel_factor(F, m, m, vars, pre, ptype, W, ldw, nb, mode, P.ustab, P.ptol, k, nc,
+                 kw, sh);
+    // rows of W below k hold L*D for the panel columns
+    trailing_update_wg(F, m, m, W + 0, ldw, kstart, k, kw);
+    (void)kstart;
+  }
+  if (tid == 0) {
+    P.npiv[p] = nc;
+    P.ndel[p] = fs - nc;
+    P.cnt[3 * p] = sh.npos;
+    P.cnt[3 * p + 1] = sh.nneg;
+    P.cnt[3 * p + 2] = sh.n2;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Solve.  Forward: front vector in LDS (post-pivot order), pulled from the
+// right-hand side and the children's update vectors.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
+                                                  double* x, int ldx, int nrhs,
+                                                  long long ustride, int chol) {
+  extern __shared__ double t[];
+  const int p = nodes[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int m = P.m[p], npiv = P.npiv[p];
+  const int ns = P.ns[p], din = P.din[p];
+  const int fs = ns + din;
+  const double* L = P.F + P.foff[p];
+  const int* vars = P.vars + P.ioff[p];
+  const int* pre = P.pre + P.ioff[p];
+  const int* ptype = P.ptype + P.ioff[p];
+  const int c0 = P.chptr[p], c1 = P.chptr[p + 1];
+  const int vp = P.vptr[p];
+  for (int r = 0; r < nrhs; r++) {
+    double* xr = x + (size_t)r * ldx;
+    for (int i = tid; i < m; i += kTPB) {
+      const int pi = i < fs ? pre[i] : i;
+      double v = pi < ns ? xr[P.node_to_var[vp + pi]] : 0.0;
+      for (int q = c0; q < c1; q++) {
+        const int c = P.ch[q];
+        ChildView cv = load_child(P, c);
+        int rc = child_row(P, cv, pi, ns, din);
+        if (rc >= 0) v += P.U[P.uoff[c] + (long long)r * ustride + (rc - cv.npiv)];
+      }
+      t[i] = v;
+    }
+    __syncthreads();
+    if (chol) {
+      for (int j = 0; j < npiv; j++) {
+        const double tj = t[j] / L[j + (size_t)j * m];
+        for (int i = j + 1 + tid; i < m; i += kTPB) t[i] -= L[i + (size_t)j * m] * tj;
+        __syncthreads();
+        if (tid == 0) t[j] = tj;
+        __syncthreads();
+      }
+    } else {
+      for (int j = 0; j < npiv;) {
+        if (ptype[j] == 1) {
+          const double tj = t[j];
+          for (int i = j + 1 + tid; i < m; i += kTPB) t[i] -= L[i + (size_t)j * m] * tj;
+          j += 1;
+        } else {
+          const double a = t[j], b = t[j + 1];
+          for (int i = j + 2 + tid; i < m; i += kTPB)
+            t[i] -= L[i + (size_t)j * m] * a + L[i + (size_t)(j + 1) * m] * b;
+          j += 2;
+        }
+        __syncthreads();
+      }
+    }
+    double* U = P.U + P.uoff[p] + (long long)r * ustride;
+    for (int i = npiv + tid; i < m; i += kTPB) U[i - npiv] = t[i];
+    if (!chol) {
+      // D solve, one thread per 1x1 pivot or 2x2 block
+      for (int j = tid; j < npiv; j += kTPB) {
+        const int ty = ptype[j];
+        if (ty == 1) {
+          t[j] = t[j] / L[j + (size_t)j * m];
+        } else if (ty == 2) {
+          const double d11 = L[j + (size_t)j * m], d21 = L[j + 1 + (size_t)j * m],
+                       d22 = L[j + 1 + (size_t)(j + 1) * m];
+          const double inv = 1.0 / (d11 * d22 - d21 * d21);
+          const double a = t[j], b = t[j + 1];
+          t[j] = inv * (d22 * a - d21 * b);
+          t[j + 1] = inv * (d11 * b - d21 * a);
+        }
+      }
+      __syncthreads();
+    }
+    for (int j = tid; j < npiv; j += kTPB) xr[unpiv(vars[j])] = t[j];
+    __syncthreads();
+  }
+}
+
+// Backward: t1 = x[pivots] - L21^T x[rows below], then L11^T back solve.
+__global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
+                                                   double* x, int ldx, int nrhs,
+                                                   int chol) {
+  extern __shared__ double t[];
+  const int p = nodes[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int m = P.m[p], npiv = P.npiv[p];
+  const int fs = P.ns[p] + P.din[p];
+  const double* L = P.F + P.foff[p];
+  const int* vars = P.vars + P.ioff[p];
+  const int* ptype = P.ptype + P.ioff[p];
+  const int cp = P.cptr[p];
+  const int w = tid / kWave, lane = tid % kWave;
+  for (int r = 0; r < nrhs; r++) {
+    double* xr = x + (size_t)r * ldx;
+    for (int i = tid; i < m; i += kTPB) {
+      int v = i < npiv ? unpiv(vars[i]) : (i < fs ? vars[i] : P.contrib_rows[cp + i - fs]);
+      t[i] = xr[v];
+    }
+    __syncthreads();
+    // t[j] -= sum_{i >= npiv} L[i,j] t[i]  (one wave per column)
+    for (int j = w; j < npiv; j += kTPB / kWave) {
+      double s = 0.0;
+      for (int i = npiv + lane; i < m; i += kWave) s += L[i + (size_t)j * m] * t[i];
+      for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if (lane == 0) t[j] -= s;
+    }
+    __syncthreads();
+    // L11^T back substitution (wave 0)
+    if (w == 0) {
+      for (int j = npiv - 1; j >= 0;) {
+        if (ptype[j] == 3) {  // 2x2 block (j-1, j)
+          double s0 = 0.0, s1 = 0.0;
+          for (int i = j + 1 + lane; i < npiv; i += kWave) {
+            s0 += L[i + (size_t)(j - 1) * m] * t[i];
+            s1 += L[i + (size_t)j * m] * t[i];
+          }
+          for (int off = kWave / 2; off > 0; off >>= 1) {
+            s0 += __shfl_xor(s0, off);
+            s1 += __shfl_xor(s1, off);
+          }
+          if (lane == 0) {
+            t[j - 1] -= s0;
+            t[j] -= s1;
+          }
+          __builtin_amdgcn_s_waitcnt(0);
+          __builtin_amdgcn_wave_barrier();
+          j -= 2;
+        } else {
+          double s = 0.0;
+          for (int i = j + 1 + lane; i < npiv; i += kWave) s += L[i + (size_t)j * m] * t[i];
+          for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+          if (lane == 0) {
+            double v = t[j] - s;
+            t[j] = chol ? v / L[j + (size_t)j * m] : v;
+          }
+          __builtin_amdgcn_s_waitcnt(0);
+          __builtin_amdgcn_wave_barrier();
+          j -= 1;
+        }
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < npiv; j += kTPB) xr[unpiv(vars[j])] = t[j];
+    __syncthreads();
+  }
+}
+
+// r = b - (K + diag(d)) x for nrhs columns; 16 lanes per row.
+__global__ void k_residual(int n, const int* rowp, const int* cols,
+                           const double* vals, const double* diag,
+                           const double* x, const double* b, double* r,
+                           int ld, int nrhs) {
+  const int lane16 = threadIdx.x & 15;
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  if (row >= n) return;
+  for (int c = 0; c < nrhs; c++) {
+    const double* xc = x + (size_t)c * ld;
+    double s = 0.0;
+    for (int p = rowp[row] + lane16; p < rowp[row + 1]; p += 16) s += vals[p] * xc[cols[p]];
+    for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+    if (lane16 == 0) {
+      double dv = diag ? diag[row] * xc[row] : 0.0;
+      r[row + (size_t)c * ld] = b[row + (size_t)c * ld] - (s + dv);
+    }
+  }
+}
+
+// sum of squares of each column into out[c] (atomic-free: one block per column)
+__global__ void k_colnorm2(int n, const double* x, int ld, double* out) {
+  __shared__ double red[kTPB / kWave];
+  const int c = blockIdx.x;
+  const double* xc = x + (size_t)c * ld;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kTPB) s += xc[i] * xc[i];
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kTPB / kWave; i++) t += red[i];
+    out[c] = t;
+  }
+}
+
+__global__ void k_axpy_cols(int n, double* x, const double* dx, int ld, int nrhs) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t tot = (size_t)n * nrhs;
+  if (i >= tot) return;
+  size_t c = i / n, r = i % n;
+  x[r + c * ld] += dx[r + c * ld];
+}
+
+// --------------------------------------------------------------------------
+// Host side.
+// --------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    HIPCHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    n = std::max<size_t>(count, 1);
+  }
+  void upload(const std::vector<T>& v) {
+    alloc(v.size());
+    if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+};
+
+struct Handle {
+  kkt_ldl_options opt;
+  int n = 0, nnz = 0;
+  std::vector<int> rowp, cols;
+  SymbolicResult S;
+  int N = 0;
+  // plan (host)
+  std::vector<int> ns, cs, cap, mode, level, lvl_ptr, lvl_nodes;
+  std::vector<long long> foff, ioff, woff, uoff, invoff;
+  std::vector<int> invnom, am_ptr, am_src, am_row;
+  std::vector<int> asm_node, asm_chunk, asm_lvl_ptr;
+  long long f_total = 0, i_total = 0, w_total = 0, u_total = 0;
+  int max_cap = 0;
+  // device
+  DBuf<int> d_ns, d_cs, d_cap, d_vptr, d_cptr, d_chptr, d_ch, d_mode, d_n2v,
+      d_crows, d_invnom, d_amptr, d_amsrc, d_amrow, d_lvl_nodes, d_asm_node,
+      d_asm_chunk, d_rowp, d_cols;
+  DBuf<long long> d_invoff, d_foff, d_ioff, d_woff, d_uoff;
+  DBuf<int> d_din, d_m, d_npiv, d_ndel, d_doff, d_stat, d_need, d_cnt;
+  DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_norm;
+  DBuf<int> d_vars, d_pre, d_ptype;
+  bool has_diag = false;
+  bool factored = false;
+  bool uploaded = false;
+  int u_nrhs = 0;
+  kkt_ldl_stats stats;
+  std::vector<int> h_npiv, h_ndel, h_cnt, h_stat;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  float last_factor_ms = 0.f, last_solve_ms = 0.f;
+};
+
+static void build_plan(Handle& H, const std::vector<int>& slack) {
+  const SymbolicResult& S = H.S;
+  const int N = S.num_nodes;
+  H.N = N;
+  H.ns.resize(N);
+  H.cs.resize(N);
+  H.cap.resize(N);
+  H.mode.resize(N);
+  for (int k = 0; k < N; k++) {
+    H.ns[k] = S.node_ptr[k + 1] - S.node_ptr[k];
+    H.cs[k] = S.contrib_ptr[k + 1] - S.contrib_ptr[k];
+    H.cap[k] = H.ns[k] + H.cs[k] + slack[k];
+    bool root = S.node_parent[k] < 0;
+    H.mode[k] = H.opt.solver_type == KKT_LDL_CHOLESKY ? 2 : (root ? 1 : 0);
+  }
+  // levels (leaves = 0)
+  H.level.assign(N, 0);
+  int nlev = 0;
+  for (int k = 0; k < N; k++) {
+    for (int q = S.children_ptr[k]; q < S.children_ptr[k + 1]; q++)
+      H.level[k] = std::max(H.level[k], H.level[S.children[q]] + 1);
+    nlev = std::max(nlev, H.level[k] + 1);
+  }
+  H.lvl_ptr.assign(nlev + 1, 0);
+  for (int k = 0; k < N; k++) H.lvl_ptr[H.level[k] + 1]++;
+  for (int l = 0; l < nlev; l++) H.lvl_ptr[l + 1] += H.lvl_ptr[l];
+  H.lvl_nodes.assign(N, 0);
+  {
+    std::vector<int> fill(H.lvl_ptr.begin(), H.lvl_ptr.end() - 1);
+    for (int k = 0; k < N; k++) H.lvl_nodes[fill[H.level[k]]++] = k;
+  }
+  // arenas
+  H.foff.resize(N);
+  H.ioff.resize(N);
+  H.woff.resize(N);
+  H.uoff.resize(N);
+  long long f = 0, iv = 0, u = 0;
+  H.max_cap = 0;
+  for (int k = 0; k < N; k++) {
+    H.foff[k] = f;
+    f += (long long)H.cap[k] * H.cap[k];
+    f = (f + 31) & ~31LL;
+    H.ioff[k] = iv;
+    iv += H.cap[k];
+    H.uoff[k] = u;
+    u += H.cap[k];
+    H.max_cap = std::max(H.max_cap, H.cap[k]);
+  }
+  H.f_total = f;
+  H.i_total = iv;
+  H.u_total = u;
+  // panel scratch: per level, nodes in the level share the arena
+  long long wmax = 0;
+  for (int l = 0; l < nlev; l++) {
+    long long w = 0;
+    for (int q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; q++) {
+      int k = H.lvl_nodes[q];
+      H.woff[k] = w;
+      w += (long long)H.cap[k] * kNB;
+    }
+    wmax = std::max(wmax, w);
+  }
+  H.w_total = wmax;
+  // assembly work items
+  H.asm_node.clear();
+  H.asm_chunk.clear();
+  H.asm_lvl_ptr.assign(nlev + 1, 0);
+  for (int l = 0; l < nlev; l++) {
+    for (int q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; q++) {
+      int k = H.lvl_nodes[q];
+      int nch = (H.cap[k] + kAsmCols - 1) / kAsmCols;
+      for (int c = 0; c < nch; c++) {
+        H.asm_node.push_back(k);
+        H.asm_chunk.push_back(c);
+      }
+    }
+    H.asm_lvl_ptr[l + 1] = (int)H.asm_node.size();
+  }
+}
+
+// Static maps that do not depend on the slack.
+static void build_maps(Handle& H) {
+  const SymbolicResult& S = H.S;
+  const int N = S.num_nodes, n = H.n;
+  std::vector<int> node_of_var(n), pos_in_node(n);
+  for (int k = 0; k < N; k++)
+    for (int q = S.node_ptr[k]; q < S.node_ptr[k + 1]; q++) {
+      node_of_var[S.node_to_var[q]] = k;
+      pos_in_node[S.node_to_var[q]] = q - S.node_ptr[k];
+    }
+  // inverse nominal maps: for child c of parent p, for every nominal
+  // position u of p (own columns then contribution rows), the index of
+  // that variable in c's contribution rows, or -1.
+  H.invoff.assign(N, -1);
+  H.invnom.clear();
+  std::vector<int> mark(n, -1);
+  for (int c = 0; c < N; c++) {
+    int p = S.node_parent[c];
+    if (p < 0) continue;
+    for (int q = S.contrib_ptr[c]; q < S.contrib_ptr[c + 1]; q++)
+      mark[S.contrib_rows[q]] = q - S.contrib_ptr[c];
+    H.invoff[c] = (long long)H.invnom.size();
+    for (int q = S.node_ptr[p]; q < S.node_ptr[p + 1]; q++)
+      H.invnom.push_back(mark[S.node_to_var[q]]);
+    for (int q = S.contrib_ptr[p]; q < S.contrib_ptr[p + 1]; q++)
+      H.invnom.push_back(mark[S.contrib_rows[q]]);
+    for (int q = S.contrib_ptr[c]; q < S.contrib_ptr[c + 1]; q++)
+      mark[S.contrib_rows[q]] = -1;
+  }
+  // original entries per own column: entries (i, var) with permuted index
+  // of i >= permuted index of var, placed at the front's nominal row.
+  H.am_ptr.assign(n + 1, 0);
+  H.am_src.clear();
+  H.am_row.clear();
+  std::vector<int> nominal(n, -1);
+  for (int k = 0; k < N; k++) {
+    const int ns = S.node_ptr[k + 1] - S.node_ptr[k];
+    for (int q = S.node_ptr[k]; q < S.node_ptr[k + 1]; q++)
+      nominal[S.node_to_var[q]] = q - S.node_ptr[k];
+    for (int q = S.contrib_ptr[k]; q < S.contrib_ptr[k + 1]; q++)
+      nominal[S.contrib_rows[q]] = ns + (q - S.contrib_ptr[k]);
+    for (int q = S.node_ptr[k]; q < S.node_ptr[k + 1]; q++) {
+      const int v = S.node_to_var[q];
+      const int pv = S.permuted ? S.iperm[v] : v;
+      for (int e = H.rowp[v]; e < H.rowp[v + 1]; e++) {
+        const int i = H.cols[e];
+        const int pi = S.permuted ? S.iperm[i] : i;
+        if (pi >= pv && nominal[i] >= 0) {
+          H.am_src.push_back(e);
+          H.am_row.push_back(nominal[i]);
+        }
+      }
+      H.am_ptr[q + 1] = (int)H.am_src.size();
+    }
+    for (int q = S.node_ptr[k]; q < S.node_ptr[k + 1]; q++) nominal[S.node_to_var[q]] = -1;
+    for (int q = S.contrib_ptr[k]; q < S.contrib_ptr[k + 1]; q++)
+      nominal[S.contrib_rows[q]] = -1;
+  }
+}
+
+static void upload_static(Handle& H) {
+  const SymbolicResult& S = H.S;
+  H.d_ns.upload(H.ns);
+  H.d_cs.upload(H.cs);
+  H.d_cap.upload(H.cap);
+  H.d_mode.upload(H.mode);
+  H.d_vptr.upload(S.node_ptr);
+  H.d_cptr.upload(S.contrib_ptr);
+  H.d_chptr.upload(S.children_ptr);
+  H.d_ch.upload(S.children);
+  H.d_n2v.upload(S.node_to_var);
+  H.d_crows.upload(S.contrib_rows);
+  H.d_invnom.upload(H.invnom);
+  H.d_invoff.upload(H.invoff);
+  H.d_amptr.upload(H.am_ptr);
+  H.d_amsrc.upload(H.am_src);
+  H.d_amrow.upload(H.am_row);
+  H.d_foff.upload(H.foff);
+  H.d_ioff.upload(H.ioff);
+  H.d_woff.upload(H.woff);
+  H.d_uoff.upload(H.uoff);
+  H.d_lvl_nodes.upload(H.lvl_nodes);
+  H.d_asm_node.upload(H.asm_node);
+  H.d_asm_chunk.upload(H.asm_chunk);
+  H.d_rowp.upload(H.rowp);
+  H.d_cols.upload(H.cols);
+  const int N = H.N;
+  H.d_din.alloc(N);
+  H.d_m.alloc(N);
+  H.d_npiv.alloc(N);
+  H.d_ndel.alloc(N);
+  H.d_doff.alloc(N);
+  H.d_stat.alloc(N);
+  H.d_need.alloc(N);
+  H.d_cnt.alloc(3 * (size_t)N);
+  H.d_F.alloc((size_t)H.f_total);
+  H.d_vars.alloc((size_t)H.i_total);
+  H.d_pre.alloc((size_t)H.i_total);
+  H.d_ptype.alloc((size_t)H.i_total);
+  H.d_W.alloc((size_t)H.w_total);
+  H.u_nrhs = 0;
+}
+
+static DevPlan dev_plan(Handle& H) {
+  DevPlan P;
+  P.ns = H.d_ns.p;
+  P.cs = H.d_cs.p;
+  P.cap = H.d_cap.p;
+  P.vptr = H.d_vptr.p;
+  P.cptr = H.d_cptr.p;
+  P.chptr = H.d_chptr.p;
+  P.ch = H.d_ch.p;
+  P.invoff = H.d_invoff.p;
+  P.foff = H.d_foff.p;
+  P.ioff = H.d_ioff.p;
+  P.woff = H.d_woff.p;
+  P.uoff = H.d_uoff.p;
+  P.mode = H.d_mode.p;
+  P.node_to_var = H.d_n2v.p;
+  P.contrib_rows = H.d_crows.p;
+  P.invnom = H.d_invnom.p;
+  P.am_ptr = H.d_amptr.p;
+  P.am_src = H.d_amsrc.p;
+  P.am_row = H.d_amrow.p;
+  P.din = H.d_din.p;
+  P.m = H.d_m.p;
+  P.npiv = H.d_npiv.p;
+  P.ndel = H.d_ndel.p;
+  P.doff = H.d_doff.p;
+  P.stat = H.d_stat.p;
+  P.need = H.d_need.p;
+  P.cnt = H.d_cnt.p;
+  P.F = H.d_F.p;
+  P.vars = H.d_vars.p;
+  P.pre = H.d_pre.p;
+  P.ptype = H.d_ptype.p;
+  P.W = H.d_W.p;
+  P.U = H.d_U.p;
+  P.vals = H.d_vals.p;
+  P.diag = H.has_diag ? H.d_diag.p : nullptr;
+  P.ustab = H.opt.ustab;
+  P.ptol = H.opt.pivot_tol;
+  return P;
+}
+
+static std::vector<int> initial_slack(const Handle& H) {
+  const SymbolicResult& S = H.S;
+  const int N = S.num_nodes;
+  std::vector<int> slack(N, 0);
+  if (H.opt.solver_type == KKT_LDL_CHOLESKY) return slack;
+  const double g = std::max(1.0, H.opt.delay_growth);
+  for (int k = 0; k < N; k++) {
+    long long below = 0;
+    for (int q = S.children_ptr[k]; q < S.children_ptr[k + 1]; q++) {
+      int c = S.children[q];
+      below += (S.node_ptr[c + 1] - S.node_ptr[c]) + slack[c];
+    }
+    int nominal = (S.node_ptr[k + 1] - S.node_ptr[k]) + (S.contrib_ptr[k + 1] - S.contrib_ptr[k]);
+    long long want = (long long)std::ceil((g - 1.0) * 0.0625 * nominal) + 4;
+    slack[k] = (int)std::min<long long>(below, want);
+  }
+  return slack;
+}
+
+static void run_factor_levels(Handle& H, hipStream_t st) {
+  DevPlan P = dev_plan(H);
+  const int nlev = (int)H.lvl_ptr.size() - 1;
+  for (int l = 0; l < nlev; l++) {
+    int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
+    hipLaunchKernelGGL(k_prepare, dim3((n1 - n0 + kTPB - 1) / kTPB), dim3(kTPB), 0, st,
+                       P, H.d_lvl_nodes.p + n0, n1 - n0);
+    int a0 = H.asm_lvl_ptr[l], a1 = H.asm_lvl_ptr[l + 1];
+    if (a1 > a0)
+      hipLaunchKernelGGL(k_assemble, dim3(a1 - a0), dim3(kTPB), 0, st, P,
+                         H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
+    hipLaunchKernelGGL(k_factor, dim3(n1 - n0), dim3(kTPB), 0, st, P,
+                       H.d_lvl_nodes.p + n0);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// Device memory is claimed on the first factorization, so that the
+// symbolic analysis (create, get_perm, get_supernodes) needs no GPU.
+static void ensure_device(Handle& H) {
+  if (H.uploaded) return;
+  upload_static(H);
+  HIPCHECK(hipEventCreate(&H.ev0));
+  HIPCHECK(hipEventCreate(&H.ev1));
+  HIPCHECK(hipEventCreate(&H.ev2));
+  HIPCHECK(hipEventCreate(&H.ev3));
+  H.uploaded = true;
+}
+
+static int factor_impl(Handle& H, const double* vals, const double* diag, int mem,
+                       hipStream_t st) {
+  auto t0 = std::chrono::steady_clock::now();
+  ensure_device(H);
+  H.d_vals.alloc(H.nnz);
+  const hipMemcpyKind kind = mem == KKT_LDL_DEVICE ? hipMemcpyDeviceToDevice
+                                                   : hipMemcpyHostToDevice;
+  HIPCHECK(hipMemcpyAsync(H.d_vals.p, vals, sizeof(double) * H.nnz, kind, st));
+  H.has_diag = diag != nullptr;
+  if (diag) {
+    H.d_diag.alloc(H.n);
+    HIPCHECK(hipMemcpyAsync(H.d_diag.p, diag, sizeof(double) * H.n, kind, st));
+  }
+  for (int attempt = 0;; attempt++) {
+    HIPCHECK(hipEventRecord(H.ev0, st));
+    run_factor_levels(H, st);
+    HIPCHECK(hipEventRecord(H.ev1, st));
+    H.h_stat.resize(H.N);
+    HIPCHECK(hipMemcpyAsync(H.h_stat.data(), H.d_stat.p, sizeof(int) * H.N,
+                            hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    bool fail = false;
+    for (int k = 0; k < H.N; k++) fail |= H.h_stat[k] != 0;
+    if (!fail) break;
+    if (attempt >= 20) throw Error(-5, "front re-planning did not converge");
+    // Grow the slack of every failed node from the observed requirement.
+    std::vector<int> need(H.N), m(H.N);
+    HIPCHECK(hipMemcpy(need.data(), H.d_need.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
+    std::vector<int> slack(H.N);
+    for (int k = 0; k < H.N; k++) {
+      int nominal = H.ns[k] + H.cs[k];
+      slack[k] = H.cap[k] - nominal;
+      if (H.h_stat[k]) {
+        int req = need[k] > 0 ? need[k] - nominal : 2 * slack[k] + 8;
+        slack[k] = std::max(slack[k], (int)std::ceil(1.25 * req) + 4);
+      }
+    }
+    build_plan(H, slack);
+    upload_static(H);
+    H.stats.replans++;
+  }
+  HIPCHECK(hipEventElapsedTime(&H.last_factor_ms, H.ev0, H.ev1));
+  H.h_npiv.resize(H.N);
+  H.h_ndel.resize(H.N);
+  H.h_cnt.resize(3 * (size_t)H.N);
+  HIPCHECK(hipMemcpy(H.h_npiv.data(), H.d_npiv.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(H.h_ndel.data(), H.d_ndel.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(H.h_cnt.data(), H.d_cnt.p, sizeof(int) * 3 * H.N, hipMemcpyDeviceToHost));
+  int nd = 0, n2 = 0;
+  long long stored = 0;
+  for (int k = 0; k < H.N; k++) {
+    nd += H.h_ndel[k];
+    n2 += H.h_cnt[3 * k + 2];
+  }
+  H.stats.num_delayed = nd;
+  H.stats.num_2x2 = n2;
+  H.stats.stored_factor = stored;
+  H.factored = true;
+  H.stats.t_factor_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+static void ensure_u(Handle& H, int nrhs) {
+  if (H.u_nrhs >= nrhs) return;
+  H.d_U.alloc((size_t)H.u_total * nrhs);
+  H.u_nrhs = nrhs;
+}
+
+static void run_solve_levels(Handle& H, double* dx, int ldx, int nrhs, hipStream_t st) {
+  ensure_u(H, nrhs);
+  DevPlan P = dev_plan(H);
+  const int nlev = (int)H.lvl_ptr.size() - 1;
+  const int chol = H.opt.solver_type == KKT_LDL_CHOLESKY;
+  size_t lds = sizeof(double) * (size_t)H.max_cap;
+  if (lds > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
+  for (int l = 0; l < nlev; l++) {
+    int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
+    hipLaunchKernelGGL(k_forward, dim3(n1 - n0), dim3(kTPB), lds, st, P,
+                       H.d_lvl_nodes.p + n0, dx, ldx, nrhs, (long long)H.u_total, chol);
+  }
+  for (int l = nlev - 1; l >= 0; l--) {
+    int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
+    hipLaunchKernelGGL(k_backward, dim3(n1 - n0), dim3(kTPB), lds, st, P,
+                       H.d_lvl_nodes.p + n0, dx, ldx, nrhs, chol);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
+                      hipStream_t st, bool refine) {
+  if (!H.factored) throw Error(-3, "solve called before factor");
+  if (nrhs <= 0) return 0;
+  if (ldx < H.n) throw Error(-1, "ldx < n");
+  const size_t tot = (size_t)ldx * nrhs;
+  double* dx = x;
+  if (mem == KKT_LDL_HOST || refine) {
+    H.d_x.alloc(tot);
+    HIPCHECK(hipMemcpyAsync(H.d_x.p, x, tot * sizeof(double),
+                            mem == KKT_LDL_HOST ? hipMemcpyHostToDevice
+                                                : hipMemcpyDeviceToDevice, st));
+    dx = H.d_x.p;
+  }
+  int status = 0;
+  HIPCHECK(hipEventRecord(H.ev2, st));
+  if (refine) {
+    H.d_b.alloc(tot);
+    H.d_r.alloc(tot);
+    H.d_norm.alloc(2 * (size_t)nrhs);
+    HIPCHECK(hipMemcpyAsync(H.d_b.p, dx, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
+    std::vector<double> bn(nrhs), rn(nrhs);
+    hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(kTPB), 0, st, H.n, H.d_b.p, ldx, H.d_norm.p);
+    HIPCHECK(hipMemcpyAsync(bn.data(), H.d_norm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    run_solve_levels(H, dx, ldx, nrhs, st);
+    int it = 0;
+    double worst = 0.0;
+    const dim3 rg((unsigned)((H.n * 16 + kTPB - 1) / kTPB));
+    for (;; it++) {
+      hipLaunchKernelGGL(k_residual, rg, dim3(kTPB), 0, st, H.n, H.d_rowp.p, H.d_cols.p,
+                         H.d_vals.p, H.has_diag ? H.d_diag.p : nullptr, dx, H.d_b.p,
+                         H.d_r.p, ldx, nrhs);
+      hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(kTPB), 0, st, H.n, H.d_r.p, ldx, H.d_norm.p);
+      HIPCHECK(hipMemcpyAsync(rn.data(), H.d_norm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      worst = 0.0;
+      for (int c = 0; c < nrhs; c++)
+        worst = std::max(worst, bn[c] > 0 ? std::sqrt(rn[c] / bn[c]) : std::sqrt(rn[c]));
+      if (worst <= H.opt.refine_tol || it >= H.opt.max_refine) break;
+      run_solve_levels(H, H.d_r.p, ldx, nrhs, st);
+      const size_t tb = ((size_t)H.n * nrhs + kTPB - 1) / kTPB;
+      hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)tb), dim3(kTPB), 0, st, H.n, dx, H.d_r.p, ldx, nrhs);
+    }
+    H.stats.refine_steps = it;
+    H.stats.residual = worst;
+    if (worst > H.opt.refine_tol) status = KKT_LDL_WARN_NOT_CONVERGED;
+  } else {
+    run_solve_levels(H, dx, ldx, nrhs, st);
+  }
+  HIPCHECK(hipEventRecord(H.ev3, st));
+  if (dx != x)
+    HIPCHECK(hipMemcpyAsync(x, dx, tot * sizeof(double),
+                            mem == KKT_LDL_HOST ? hipMemcpyDeviceToHost
+                                                : hipMemcpyDeviceToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  HIPCHECK(hipEventElapsedTime(&H.last_solve_ms, H.ev2, H.ev3));
+  return status;
+}
+
+static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
+                        const kkt_ldl_options* o) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (o) H.opt = *o;
+  else kkt_ldl_default_options(&H.opt);
+  if (n <= 0) throw Error(-1, "n must be positive");
+  H.n = n;
+  H.nnz = rowp[n];
+  H.rowp.assign(rowp, rowp + n + 1);
+  H.cols.assign(cols, cols + H.nnz);
+  for (int i = 0; i < n; i++) {
+    if (rowp[i + 1] < rowp[i]) throw Error(-1, "rowp not monotone");
+    for (int p = rowp[i]; p < rowp[i + 1]; p++) {
+      if (cols[p] < 0 || cols[p] >= n) throw Error(-1, "column index out of range");
+      if (p > rowp[i] && cols[p] <= cols[p - 1]) throw Error(-1, "columns not sorted/unique");
+    }
+  }
+  std::vector<int> perm;
+  const int* pp = nullptr;
+  if (H.opt.order == KKT_LDL_ORDER_AMD) {
+    perm.resize(n);
+    amd_ordering(n, rowp, cols, perm.data());
+    pp = perm.data();
+  } else if (H.opt.order == KKT_LDL_ORDER_ND) {
+    perm.resize(n);
+    nd_ordering(n, rowp, cols, perm.data());
+    pp = perm.data();
+  } else if (H.opt.order == KKT_LDL_ORDER_GIVEN) {
+    if (!H.opt.perm) throw Error(-1, "ORDER_GIVEN without perm");
+    perm.assign(H.opt.perm, H.opt.perm + n);
+    std::vector<char> seen(n, 0);
+    for (int v : perm) {
+      if (v < 0 || v >= n || seen[v]) throw Error(-1, "perm is not a permutation");
+      seen[v] = 1;
+    }
+    pp = perm.data();
+  }
+  analyze(n, rowp, cols, pp, H.S);
+  H.opt.perm = nullptr;
+  build_maps(H);
+  build_plan(H, initial_slack(H));
+  std::memset(&H.stats, 0, sizeof(H.stats));
+  H.stats.n = n;
+  H.stats.num_nodes = H.N;
+  H.stats.num_snodes = H.S.num_snodes;
+  H.stats.num_levels = (int)H.lvl_ptr.size() - 1;
+  int mf = 0;
+  for (int k = 0; k < H.N; k++) mf = std::max(mf, H.ns[k] + H.cs[k]);
+  H.stats.max_front = mf;
+  double fl = 0.0;
+  long long nz = 0;
+  for (int j = 0; j < n; j++) {
+    double c = H.S.colcount[j] - 1;
+    fl += c * (c + 2.0);
+    nz += H.S.colcount[j];
+  }
+  H.stats.flops_factor = fl;
+  H.stats.nnz_l = nz;
+  H.stats.t_analyze_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace kktldl
+
+using kktldl::Error;
+using kktldl::Handle;
+
+struct kkt_ldl_handle {
+  Handle h;
+};
+
+template <class Fn>
+static int guarded(Fn&& fn) {
+  try {
+    return fn();
+  } catch (const Error& e) {
+    kktldl::g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    kktldl::g_last_error = e.what();
+    return -100;
+  } catch (...) {
+    kktldl::g_last_error = "unknown error";
+    return -101;
+  }
+}
+
+extern "C" {
+
+void kkt_ldl_default_options(kkt_ldl_options* o) {
+  o->order = KKT_LDL_ORDER_AMD;
+  o->solver_type = KKT_LDL_LDL;
+  o->ustab = 0.01;
+  o->pivot_tol = 1e-14;
+  o->delay_growth = 2.0;
+  o->precision = KKT_LDL_FP64;
+  o->refine_tol = 1e-10;
+  o->max_refine = 10;
+  o->perm = nullptr;
+}
+
+int kkt_ldl_create(int n, const int* rowp, const int* cols,
+                   const kkt_ldl_options* opts, kkt_ldl_t* out) {
+  return guarded([&] {
+    if (!out || !rowp || !cols) throw Error(-1, "null argument");
+    std::unique_ptr<kkt_ldl_handle> h(new kkt_ldl_handle());
+    kktldl::create_impl(h->h, n, rowp, cols, opts);
+    *out = h.release();
+    return 0;
+  });
+}
+
+int kkt_ldl_factor(kkt_ldl_t h, const double* vals, const double* diag, int mem,
+                   void* stream) {
+  return guarded([&] {
+    if (!h || !vals) throw Error(-1, "null argument");
+    return kktldl::factor_impl(h->h, vals, diag, mem, (hipStream_t)stream);
+  });
+}
+
+int kkt_ldl_solve(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem, void* stream) {
+  return guarded([&] {
+    if (!h || !x) throw Error(-1, "null argument");
+    return kktldl::solve_impl(h->h, nrhs, x, ldx, mem, (hipStream_t)stream, false);
+  });
+}
+
+int kkt_ldl_solve_refined(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem,
+                          void* stream) {
+  return guarded([&] {
+    if (!h || !x) throw Error(-1, "null argument");
+    return kktldl::solve_impl(h->h, nrhs, x, ldx, mem, (hipStream_t)stream, true);
+  });
+}
+
+int kkt_ldl_inertia(kkt_ldl_t h, int* npos, int* nneg, int* nzero) {
+  return guarded([&] {
+    if (!h || !npos || !nneg) throw Error(-1, "null argument");
+    Handle& H = h->h;
+    if (!H.factored) throw Error(-3, "inertia requested before factor");
+    int a = 0, b = 0;
+    if (H.opt.solver_type != KKT_LDL_CHOLESKY)
+      for (int k = 0; k < H.N; k++) {
+        a += H.h_cnt[3 * k];
+        b += H.h_cnt[3 * k + 1];
+      }
+    *npos = a;
+    *nneg = b;
+    if (nzero) *nzero = 0;
+    return 0;
+  });
+}
+
+int kkt_ldl_get_perm(kkt_ldl_t h, int* perm, int* iperm) {
+  return guarded([&] {
+    if (!h || !perm || !iperm) throw Error(-1, "null argument");
+    const auto& S = h->h.S;
+    for (int i = 0; i < S.n; i++) {
+      perm[i] = S.permuted ? S.perm[i] : i;
+      iperm[i] = S.permuted ? S.iperm[i] : i;
+    }
+    return 0;
+  });
+}
+
+int kkt_ldl_get_sizes(kkt_ldl_t h, int* num_nodes, int* contrib_total) {
+  return guarded([&] {
+    if (!h) throw Error(-1, "null argument");
+    if (num_nodes) *num_nodes = h->h.S.num_nodes;
+    if (contrib_total) *contrib_total = h->h.S.contrib_ptr[h->h.S.num_nodes];
+    return 0;
+  });
+}
+
+int kkt_ldl_get_supernodes(kkt_ldl_t h, int* node_ptr, int* node_to_var,
+                           int* children_ptr, int* children, int* contrib_ptr,
+                           int* contrib_rows) {
+  return guarded([&] {
+    if (!h) throw Error(-1, "null argument");
+    const auto& S = h->h.S;
+    auto cp = [](const std::vector<int>& v, int* o) {
+      if (o) std::copy(v.begin(), v.end(), o);
+    };
+    cp(S.node_ptr, node_ptr);
+    cp(S.node_to_var, node_to_var);
+    cp(S.children_ptr, children_ptr);
+    cp(S.children, children);
+    cp(S.contrib_ptr, contrib_ptr);
+    cp(S.contrib_rows, contrib_rows);
+    return 0;
+  });
+}
+
+int kkt_ldl_get_node_info(kkt_ldl_t h, int* npiv, int* ndel, int* n2x2, int* root) {
+  return guarded([&] {
+    if (!h) throw Error(-1, "null argument");
+    Handle& H = h->h;
+    if (!H.factored) throw Error(-3, "node info requested before factor");
+    for (int k = 0; k < H.N; k++) {
+      if (npiv) npiv[k] = H.h_npiv[k];
+      if (ndel) ndel[k] = H.h_ndel[k];
+      if (n2x2) n2x2[k] = H.h_cnt[3 * k + 2];
+      if (root) root[k] = H.S.node_parent[k] < 0;
+    }
+    return 0;
+  });
+}
+
+int kkt_ldl_get_stats(kkt_ldl_t h, kkt_ldl_stats* st) {
+  return guarded([&] {
+    if (!h || !st) throw Error(-1, "null argument");
+    h->h.stats.front_bytes = h->h.f_total * (long long)sizeof(double);
+    *st = h->h.stats;
+    return 0;
+  });
+}
+
+int kkt_ldl_last_times(kkt_ldl_t h, double* factor_ms, double* solve_ms) {
+  return guarded([&] {
+    if (!h) throw Error(-1, "null argument");
+    if (factor_ms) *factor_ms = h->h.last_factor_ms;
+    if (solve_ms) *solve_ms = h->h.last_solve_ms;
+    return 0;
+  });
+}
+
+const char* kkt_ldl_last_error(void) { return kktldl::g_last_error.c_str(); }
+
+void kkt_ldl_destroy(kkt_ldl_t h) {
+  if (!h) return;
+  Handle& H = h->h;
+  if (H.ev0) (void)hipEventDestroy(H.ev0);
+  if (H.ev1) (void)hipEventDestroy(H.ev1);
+  if (H.ev2) (void)hipEventDestroy(H.ev2);
+  if (H.ev3) (void)hipEventDestroy(H.ev3);
+  delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// Host-side symbolic analysis of the KKT pattern: fill-reducing ordering,
+// elimination tree, supernodes and the per-node row structure.  The results
+// are bit-identical to the reference's SparseLDL constructor
+// (include/sparse_ldl.h:2222-2376) for NATURAL and AMD orderings, because
+// the numeric factor and the solve index everything by this structure.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace kktldl {
+
+enum Ordering { ORDER_NATURAL = 0, ORDER_AMD = 1, ORDER_ND = 2, ORDER_GIVEN = 3 };
+
+struct SymbolicResult {
+  int n = 0;
+  bool permuted = false;
+  std::vector<int> perm, iperm;       // perm[new] = old
+  std::vector<int> parent, post;      // elimination tree (permuted space)
+  std::vector<int> colcount;          // fundamental column counts incl. diag
+  int num_snodes = 0;                 // fundamental supernodes before merging
+  int num_nodes = 0;                  // relaxed supernodes
+  std::vector<int> node_ptr;          // [N+1]
+  std::vector<int> node_to_var;       // [n], original indices
+  std::vector<int> node_parent;       // [N], -1 for roots
+  std::vector<int> children_ptr;      // [N+2], roots after position N
+  std::vector<int> children;          // [N]
+  std::vector<int> contrib_ptr;       // [N+1]
+  std::vector<int> contrib_rows;      // original indices, ascending permuted order
+};
+
+// Reference BlockAMD (include/block_amd.h:27-452) with no multiplier
+// constraint, on the pattern with the diagonal added (ordering_utils.h:313-360).
+void amd_ordering(int n, const int* rowp, const int* cols, int* perm);
+
+// Nested dissection on the graph of the pattern (no METIS in this build):
+// multilevel-free recursive bisection with BFS level-set separators, leaves
+// ordered by AMD.  perm[new] = old.
+void nd_ordering(int n, const int* rowp, const int* cols, int* perm);
+
+// Full symbolic analysis.  `perm` may be null (natural order).
+void analyze(int n, const int* rowp, const int* cols, const int* perm,
+             SymbolicResult& S);
+
+}  // namespace kktldl

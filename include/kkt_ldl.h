@@ -1,0 +1,133 @@
+/*
+ * kkt_ldl.h -- C ABI of the MI355X supernodal LDL^T engine for interior-point
+ * KKT systems.  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Every call returns an int status: 0 ok, < 0 error (message from
+ * kkt_ldl_last_error(), thread-local), > 0 warning (see KKT_LDL_WARN_*).
+ * No C++ exception crosses this boundary.
+ *
+ * Threading: one handle per thread.  kkt_ldl_solve may be called
+ * concurrently on one handle from one thread's stream only; kkt_ldl_factor
+ * may not overlap any other call on the same handle.
+ *
+ * Each entry point replaces a piece of the reference's SparseLDL binding
+ * (amigo/amigo.cpp:648-732 over include/sparse_ldl.h); the reference file and
+ * line is given next to each declaration.
+ */
+#ifndef KKT_LDL_H
+#define KKT_LDL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kkt_ldl_handle* kkt_ldl_t;
+
+/* Orderings (include/ordering_utils.h:22 OrderingType). */
+enum { KKT_LDL_ORDER_NATURAL = 0, KKT_LDL_ORDER_AMD = 1, KKT_LDL_ORDER_ND = 2,
+       KKT_LDL_ORDER_GIVEN = 3 };
+/* Factorization kinds (include/sparse_ldl.h:23 SolverType). */
+enum { KKT_LDL_LDL = 0, KKT_LDL_CHOLESKY = 1 };
+/* Arithmetic of the stored factor. */
+enum { KKT_LDL_FP64 = 0, KKT_LDL_FP32_REFINE = 1 };
+/* Memory space of caller buffers. */
+enum { KKT_LDL_HOST = 0, KKT_LDL_DEVICE = 1 };
+/* Positive status codes (warnings). */
+enum { KKT_LDL_WARN_SINGULAR = 1, KKT_LDL_WARN_NOT_CONVERGED = 2 };
+
+typedef struct {
+  int order;            /* KKT_LDL_ORDER_*                                  */
+  int solver_type;      /* KKT_LDL_LDL or KKT_LDL_CHOLESKY                  */
+  double ustab;         /* threshold-pivot stability factor (ref. 0.01)     */
+  double pivot_tol;     /* absolute pivot tolerance (ref. 1e-14)            */
+  double delay_growth;  /* initial front slack for delayed pivots (ref. 2)  */
+  int precision;        /* KKT_LDL_FP64 or KKT_LDL_FP32_REFINE              */
+  double refine_tol;    /* ||Kx-b||/||b|| target of kkt_ldl_solve_refined   */
+  int max_refine;       /* refinement step limit                            */
+  const int* perm;      /* KKT_LDL_ORDER_GIVEN: perm[new] = old             */
+} kkt_ldl_options;
+
+typedef struct {
+  int n, num_nodes, num_snodes, num_levels, max_front;
+  long long nnz_l;          /* fundamental nnz(L) incl. diagonal            */
+  double flops_factor;      /* fundamental sum c_j (c_j + 2)                */
+  long long stored_factor;  /* executed: sum over nodes rows x pivots       */
+  int num_delayed;          /* delayed pivots summed over nodes (last factor)*/
+  int num_2x2;              /* 2x2 pivot blocks (last factor)               */
+  int replans;              /* front re-layouts caused by delayed pivots    */
+  int refine_steps;         /* last kkt_ldl_solve_refined                   */
+  double residual;          /* last relative residual ||Kx-b||/||b||        */
+  double t_analyze_ms, t_factor_ms, t_solve_ms;
+  long long front_bytes;    /* device arena holding every front             */
+} kkt_ldl_stats;
+
+/* Defaults matching AmigoSolver (amigo/algorithm/solvers/amigo_solver.py:15-25). */
+void kkt_ldl_default_options(kkt_ldl_options* opts);
+
+/* Pattern + symbolic analysis.  rowp[n+1], cols[nnz] (symmetric, sorted,
+ * diagonal present).  Replaces SparseLDL(mat, solver_type, ustab, pivot_tol,
+ * delay_growth, order) -- include/sparse_ldl.h:65-118, amigo/amigo.cpp:655-662. */
+int kkt_ldl_create(int n, const int* rowp, const int* cols,
+                   const kkt_ldl_options* opts, kkt_ldl_t* out);
+
+/* Numeric factorization of K + diag(d).  vals[nnz] in CSR order, diag[n] or
+ * NULL.  mem = KKT_LDL_HOST or KKT_LDL_DEVICE for both pointers; stream is a
+ * hipStream_t (NULL = default stream).  Replaces SparseLDL.factor(diagonal)
+ * -- include/sparse_ldl.h:136-158, amigo/amigo.cpp:663-664. */
+int kkt_ldl_factor(kkt_ldl_t h, const double* vals, const double* diag, int mem,
+                   void* stream);
+
+/* x <- (K + diag d)^{-1} x for nrhs columns of length n with leading
+ * dimension ldx (column major, FP64).  Replaces SparseLDL.solve -- include/
+ * sparse_ldl.h:169-197, amigo/amigo.cpp:665-727. */
+int kkt_ldl_solve(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem,
+                  void* stream);
+
+/* Solve followed by FP64 iterative refinement on the GPU against the stored
+ * K + diag until ||Kx-b||/||b|| <= refine_tol (or max_refine steps).  No
+ * counterpart in the reference (it never refines, amigo/algorithm/solvers/
+ * linear_solver.py:5-35); used by the FP32 factor and by the IPM drop-in.
+ * Returns KKT_LDL_WARN_NOT_CONVERGED if the target is not met. */
+int kkt_ldl_solve_refined(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem,
+                          void* stream);
+
+/* Inertia from D of the last factorization; zero pivots count positive as in
+ * the reference.  Replaces SparseLDL.get_inertia -- include/sparse_ldl.h:
+ * 205-213, 1935-1987, amigo/amigo.cpp:728-731.  nzero may be NULL. */
+int kkt_ldl_inertia(kkt_ldl_t h, int* npos, int* nneg, int* nzero);
+
+/* Fill-reducing permutation (perm[new] = old, iperm[old] = new); identity for
+ * NATURAL.  Exposes SparseLDL::invperm (include/sparse_ldl.h:3265). */
+int kkt_ldl_get_perm(kkt_ldl_t h, int* perm, int* iperm);
+
+/* Number of relaxed supernodes and total contribution rows, to size the
+ * arrays of kkt_ldl_get_supernodes. */
+int kkt_ldl_get_sizes(kkt_ldl_t h, int* num_nodes, int* contrib_total);
+
+/* Supernode partition and tree exactly as SparseLDL stores them
+ * (node_ptr[N+1], node_to_var[n], children_ptr[N+2], children[N],
+ * contrib_ptr[N+1], contrib_rows[contrib_total]) -- include/sparse_ldl.h:
+ * 3268-3281. */
+int kkt_ldl_get_supernodes(kkt_ldl_t h, int* node_ptr, int* node_to_var,
+                           int* children_ptr, int* children, int* contrib_ptr,
+                           int* contrib_rows);
+
+/* Per-node pivot outcome of the last factorization: pivots, delayed-out
+ * count, 2x2 blocks, root flag (arrays of num_nodes). */
+int kkt_ldl_get_node_info(kkt_ldl_t h, int* npiv, int* ndel, int* n2x2,
+                          int* root);
+
+int kkt_ldl_get_stats(kkt_ldl_t h, kkt_ldl_stats* st);
+
+/* Device time of the last factor / solve kernels on the caller's stream, ms
+ * (hipEvent based). */
+int kkt_ldl_last_times(kkt_ldl_t h, double* factor_ms, double* solve_ms);
+
+const char* kkt_ldl_last_error(void);
+
+void kkt_ldl_destroy(kkt_ldl_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KKT_LDL_H */
