@@ -66,10 +66,19 @@ constexpr int kNB = 64;        // panel width bound (reference block_size)
 constexpr int kTPB = 256;      // threads per workgroup
 constexpr int kWave = 64;
 constexpr int kAsmCols = 8;    // columns per assembly workgroup
+constexpr int kLargeFront = 256;  // fronts above this use k_panel + k_update
+constexpr int kPanelThreads = 1024;  // workgroup of the large-front panel
+constexpr int kLdsFront = 96;   // default bound of fronts assembled and factored in LDS
 
 // --------------------------------------------------------------------------
 // Device-side plan (pointers into device memory).
 // --------------------------------------------------------------------------
+// Factorization state of a large front between panel launches.
+struct PanelState {
+  int k, nc, kstart, kw;
+  int npos, nneg, n2, pad;
+};
+
 struct DevPlan {
   // static per node
   const int* ns;       // own (fully summed nominal) columns
@@ -100,6 +109,7 @@ struct DevPlan {
   int* stat;           // 0 ok, 1 overflow / failed
   int* need;           // front dimension required when overflowing
   int* cnt;            // [3N]: npos, nneg, n2x2
+  PanelState* ps;      // [N]
   // arenas
   double* F;
   int* vars;
@@ -177,6 +187,13 @@ __global__ void k_prepare(DevPlan P, const int* nodes, int count) {
   } else {
     P.stat[p] = 0;
   }
+  PanelState st;
+  st.k = 0;
+  st.nc = P.mode[p] == 1 ? m : P.ns[p] + din;
+  st.kstart = 0;
+  st.kw = 0;
+  st.npos = st.nneg = st.n2 = st.pad = 0;
+  P.ps[p] = st;
 }
 
 // --------------------------------------------------------------------------
@@ -283,15 +300,17 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
 // --------------------------------------------------------------------------
 // Workgroup reductions.
 // --------------------------------------------------------------------------
+template <int NT>
 struct RedBuf {
-  double v[kTPB / kWave];
-  int i[kTPB / kWave];
-  double v2[kTPB / kWave];
+  double v[NT / kWave];
+  int i[NT / kWave];
+  double v2[NT / kWave];
 };
 
 // Block-wide (max |x|, first index) and second plain max; results broadcast.
+template <int NT>
 __device__ inline void block_argmax2(double& val, int& idx, double& val2,
-                                     RedBuf& rb) {
+                                     RedBuf<NT>& rb) {
   for (int off = kWave / 2; off > 0; off >>= 1) {
     double ov = __shfl_xor(val, off);
     int oi = __shfl_xor(idx, off);
@@ -313,7 +332,7 @@ __device__ inline void block_argmax2(double& val, int& idx, double& val2,
   val = rb.v[0];
   idx = rb.i[0];
   val2 = rb.v2[0];
-  for (int t = 1; t < kTPB / kWave; t++) {
+  for (int t = 1; t < NT / kWave; t++) {
     if (rb.v[t] > val || (rb.v[t] == val && rb.i[t] < idx)) {
       val = rb.v[t];
       idx = rb.i[t];
@@ -326,13 +345,15 @@ __device__ inline void block_argmax2(double& val, int& idx, double& val2,
 // Front factorization (one workgroup).  F: front (ld = m), W: panel
 // workspace (ld = ldw, kNB columns).
 // --------------------------------------------------------------------------
+template <int NT>
 struct FrontShared {
   double wr[kNB];
   double wr2[kNB];
-  RedBuf rb;
+  RedBuf<NT> rb;
   int npos, nneg, n2;
 };
 
+template <int NT>
 __device__ inline void sym_swap(double* F, int ld, int m, int* vars, int* pre,
                                 int a, int b) {
   const int tid = threadIdx.x;
@@ -347,17 +368,17 @@ __device__ inline void sym_swap(double* F, int ld, int m, int* vars, int* pre,
     F[a + (size_t)a * ld] = F[b + (size_t)b * ld];
     F[b + (size_t)b * ld] = d;
   }
-  for (int j = tid; j < a; j += kTPB) {
+  for (int j = tid; j < a; j += NT) {
     double t = F[a + (size_t)j * ld];
     F[a + (size_t)j * ld] = F[b + (size_t)j * ld];
     F[b + (size_t)j * ld] = t;
   }
-  for (int j = a + 1 + tid; j < b; j += kTPB) {
+  for (int j = a + 1 + tid; j < b; j += NT) {
     double t = F[j + (size_t)a * ld];
     F[j + (size_t)a * ld] = F[b + (size_t)j * ld];
     F[b + (size_t)j * ld] = t;
   }
-  for (int i = b + 1 + tid; i < m; i += kTPB) {
+  for (int i = b + 1 + tid; i < m; i += NT) {
     double t = F[i + (size_t)a * ld];
     F[i + (size_t)a * ld] = F[i + (size_t)b * ld];
     F[i + (size_t)b * ld] = t;
@@ -368,22 +389,23 @@ __device__ inline void sym_swap(double* F, int ld, int m, int* vars, int* pre,
 // nc advanced; W[k0:, 0:kw] and F[:, kstart:k] hold the panel for the
 // trailing update.  mode 0: threshold pivoting with delays; mode 1:
 // Bunch-Kaufman (every column is a candidate, nothing is delayed).
+template <int NT>
 __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
                              int* ptype, double* W, int ldw, int nb, int mode,
                              double ustab,
                              double ptol, int& k, int& nc, int& kw_out,
-                             FrontShared& sh) {
+                             FrontShared<NT>& sh) {
   const int tid = threadIdx.x;
   const double alpha = 0.6403882032022076;  // (1 + sqrt(17)) / 8
   const int kstart = k;
   int kw = 0;
   while (kw < nb - 1 && k < nc) {
     // W[k:m, kw] = F[k:m, k] - F[k:m, kstart:k] W[k, 0:kw]^T
-    for (int c = tid; c < kw; c += kTPB) sh.wr[c] = W[k + (size_t)c * ldw];
+    for (int c = tid; c < kw; c += NT) sh.wr[c] = W[k + (size_t)c * ldw];
     __syncthreads();
     double lbest = -1.0, lrest = 0.0;
     int lidx = 0x7fffffff;
-    for (int i = k + tid; i < m; i += kTPB) {
+    for (int i = k + tid; i < m; i += NT) {
       double v = F[i + (size_t)k * ld];
       for (int c = 0; c < kw; c++) v -= F[i + (size_t)(kstart + c) * ld] * sh.wr[c];
       W[i + (size_t)kw * ldw] = v;
@@ -397,7 +419,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
         lrest = fmax(lrest, a);
       }
     }
-    block_argmax2(lbest, lidx, lrest, sh.rb);
+    block_argmax2<NT>(lbest, lidx, lrest, sh.rb);
     const double awkk = fabs(W[k + (size_t)kw * ldw]);
     int imax = 0;
     double awik = 0.0;
@@ -431,11 +453,11 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
     }
     if (need_row) {
       // W[k:m, kw+1] = column imax updated by the panel so far
-      for (int c = tid; c < kw; c += kTPB) sh.wr2[c] = W[imax + (size_t)c * ldw];
+      for (int c = tid; c < kw; c += NT) sh.wr2[c] = W[imax + (size_t)c * ldw];
       __syncthreads();
       double* W1 = W + (size_t)(kw + 1) * ldw;
       double lrow = 0.0;
-      for (int i = k + tid; i < m; i += kTPB) {
+      for (int i = k + tid; i < m; i += NT) {
         double v = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
         for (int c = 0; c < kw; c++) v -= F[i + (size_t)(kstart + c) * ld] * sh.wr2[c];
         W1[i] = v;
@@ -443,7 +465,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
       }
       double dummy = -1.0;
       int di = 0;
-      block_argmax2(dummy, di, lrow, sh.rb);
+      block_argmax2<NT>(dummy, di, lrow, sh.rb);
       const double rowmax = lrow;
       const double awii = fabs(W1[imax]);
       if (mode == 1) {
@@ -474,14 +496,14 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
         }
       }
       if (step == 1 && kpiv == imax) {
-        for (int i = k + tid; i < m; i += kTPB) W[i + (size_t)kw * ldw] = W1[i];
+        for (int i = k + tid; i < m; i += NT) W[i + (size_t)kw * ldw] = W1[i];
       }
       __syncthreads();
     }
     if (step == 0) {  // delay candidate k (threshold mode only)
       if (k < nc - 1) {
-        sym_swap(F, ld, m, vars, pre, k, nc - 1);
-        for (int c = tid; c < kw; c += kTPB) {
+        sym_swap<NT>(F, ld, m, vars, pre, k, nc - 1);
+        for (int c = tid; c < kw; c += NT) {
           double t = W[k + (size_t)c * ldw];
           W[k + (size_t)c * ldw] = W[nc - 1 + (size_t)c * ldw];
           W[nc - 1 + (size_t)c * ldw] = t;
@@ -493,8 +515,8 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
     }
     const int ksrc = k + step - 1;
     if (kpiv != ksrc) {
-      sym_swap(F, ld, m, vars, pre, ksrc, kpiv);
-      for (int c = tid; c < kw + step; c += kTPB) {
+      sym_swap<NT>(F, ld, m, vars, pre, ksrc, kpiv);
+      for (int c = tid; c < kw + step; c += NT) {
         double t = W[ksrc + (size_t)c * ldw];
         W[ksrc + (size_t)c * ldw] = W[kpiv + (size_t)c * ldw];
         W[kpiv + (size_t)c * ldw] = t;
@@ -504,7 +526,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
     if (step == 1) {
       const double d = W[k + (size_t)kw * ldw];
       const double dinv = (zero_col || d == 0.0) ? 0.0 : 1.0 / d;
-      for (int i = k + tid; i < m; i += kTPB) {
+      for (int i = k + tid; i < m; i += NT) {
         double v = W[i + (size_t)kw * ldw];
         F[i + (size_t)k * ld] = (i == k || zero_col) ? v : v * dinv;
       }
@@ -520,7 +542,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
       const double det = d11 * d22 - d21 * d21;
       const double detinv = 1.0 / det;
       const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
-      for (int i = k + 2 + tid; i < m; i += kTPB) {
+      for (int i = k + 2 + tid; i < m; i += NT) {
         double a = W[i + (size_t)kw * ldw], b = W[i + (size_t)(kw + 1) * ldw];
         F[i + (size_t)k * ld] = a * i00 + b * i01;
         F[i + (size_t)(k + 1) * ld] = a * i01 + b * i11;
@@ -552,13 +574,14 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
 
 // F[k:m, k:m] (lower) -= F[k:m, kstart:kstart+kw] * W[k:m, 0:kw]^T, one
 // workgroup, 4x4 register tiles.
+template <int NT>
 __device__ void trailing_update_wg(double* F, int ld, int m, const double* W,
                                    int ldw, int kstart, int k, int kw) {
   const int nt = m - k;
   if (nt <= 0 || kw == 0) return;
   const int nb4 = (nt + 3) / 4;
   const int ntiles = nb4 * (nb4 + 1) / 2;
-  for (int t = threadIdx.x; t < ntiles; t += kTPB) {
+  for (int t = threadIdx.x; t < ntiles; t += NT) {
     // map t -> (bi >= bj)
     int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while (bi * (bi + 1) / 2 > t) bi--;
@@ -597,16 +620,17 @@ __device__ void trailing_update_wg(double* F, int ld, int m, const double* W,
 
 // Cholesky mode (include/sparse_ldl.h:2010-2044): right-looking, one
 // workgroup; used by the reference's tests only.
+template <int NT>
 __device__ void cholesky_front(double* F, int ld, int m, int fs) {
   const int tid = threadIdx.x;
   for (int k = 0; k < fs; k++) {
     const double d = sqrt(F[k + (size_t)k * ld]);
     __syncthreads();
-    for (int i = k + 1 + tid; i < m; i += kTPB) F[i + (size_t)k * ld] /= d;
+    for (int i = k + 1 + tid; i < m; i += NT) F[i + (size_t)k * ld] /= d;
     if (tid == 0) F[k + (size_t)k * ld] = d;
     __syncthreads();
     const int nt = m - k - 1;
-    for (int e = tid; e < nt * nt; e += kTPB) {
+    for (int e = tid; e < nt * nt; e += NT) {
       int jj = e / nt, ii = e % nt;
       if (ii < jj) continue;
       int i = k + 1 + ii, j = k + 1 + jj;
@@ -617,9 +641,10 @@ __device__ void cholesky_front(double* F, int ld, int m, int fs) {
 }
 
 __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
+  constexpr int NT = kTPB;
   const int p = nodes[blockIdx.x];
   if (P.stat[p]) return;
-  __shared__ FrontShared sh;
+  __shared__ FrontShared<NT> sh;
   const int tid = threadIdx.x;
   const int m = P.m[p];
   const int fs = P.ns[p] + P.din[p];
@@ -637,8 +662,8 @@ __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
   }
   __syncthreads();
   if (mode == 2) {
-    cholesky_front(F, m, m, fs);
-    for (int j = tid; j < fs; j += kTPB) ptype[j] = 1;
+    cholesky_front<NT>(F, m, m, fs);
+    for (int j = tid; j < fs; j += NT) ptype[j] = 1;
     if (tid == 0) {
       P.npiv[p] = fs;
       P.ndel[p] = 0;
@@ -655,10 +680,10 @@ __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
   while (k < nc) {
     const int kstart = k;
     int kw = 0;
-    panel_factor(F, m, m, vars, pre, ptype, W, ldw, nb, mode, P.ustab, P.ptol, k, nc,
-                 kw, sh);
+    panel_factor<NT>(F, m, m, vars, pre, ptype, W, ldw, nb, mode, P.ustab, P.ptol, k, nc,
+                     kw, sh);
     // rows of W below k hold L*D for the panel columns
-    trailing_update_wg(F, m, m, W + 0, ldw, kstart, k, kw);
+    trailing_update_wg<NT>(F, m, m, W, ldw, kstart, k, kw);
     (void)kstart;
   }
   if (tid == 0) {
@@ -668,6 +693,219 @@ __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
     P.cnt[3 * p + 1] = sh.nneg;
     P.cnt[3 * p + 2] = sh.n2;
   }
+}
+
+// --------------------------------------------------------------------------
+// Small fronts: assembled straight into LDS, factored there, written back
+// once.  LDS layout: F (m x m, ld m) then W (m x kNB, ld m).
+// --------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
+  extern __shared__ double lds[];
+  __shared__ FrontShared<NT> sh;
+  __shared__ ChildView s_cv[kChildCache];
+  const int p = nodes[blockIdx.x];
+  if (P.stat[p]) return;
+  const int tid = threadIdx.x;
+  const int m = P.m[p], ns = P.ns[p], cs = P.cs[p], din = P.din[p];
+  const int fs = ns + din, mode = P.mode[p];
+  const int c0 = P.chptr[p], nch = P.chptr[p + 1] - c0;
+  const bool cached = nch <= kChildCache;
+  if (cached)
+    for (int t = tid; t < nch; t += NT) s_cv[t] = load_child(P, P.ch[c0 + t]);
+  int* vars = P.vars + P.ioff[p];
+  int* pre = P.pre + P.ioff[p];
+  int* ptype = P.ptype + P.ioff[p];
+  {
+    const int vp = P.vptr[p], cp = P.cptr[p];
+    for (int i = tid; i < ns; i += NT) vars[i] = P.node_to_var[vp + i];
+    for (int i = tid; i < cs; i += NT) vars[fs + i] = P.contrib_rows[cp + i];
+    for (int i = tid; i < m; i += NT) pre[i] = i;
+    for (int q = 0; q < nch; q++) {
+      const int c = P.ch[c0 + q];
+      const int nd = P.ndel[c], off = P.doff[c];
+      const int* cv = P.vars + P.ioff[c] + P.npiv[c];
+      for (int t = tid; t < nd; t += NT) vars[ns + off + t] = cv[t];
+    }
+  }
+  if (tid == 0) {
+    sh.npos = 0;
+    sh.nneg = 0;
+    sh.n2 = 0;
+  }
+  __syncthreads();
+  double* F = lds;
+  double* W = lds + (size_t)m * m;
+  // pull the children's contributions into the lower triangle
+  for (int e = tid; e < m * m; e += NT) {
+    const int j = e / m, i = e - j * m;
+    double v = 0.0;
+    if (i >= j) {
+      for (int q = 0; q < nch; q++) {
+        ChildView cv = cached ? s_cv[q] : load_child(P, P.ch[c0 + q]);
+        const int rj = child_row(P, cv, j, ns, din);
+        if (rj < 0) continue;
+        const int ri = child_row(P, cv, i, ns, din);
+        if (ri >= 0) v += cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
+      }
+    }
+    F[e] = v;
+  }
+  __syncthreads();
+  const int vp = P.vptr[p];
+  for (int j = 0; j < ns; j++) {
+    const int g = vp + j;
+    for (int e = P.am_ptr[g] + tid; e < P.am_ptr[g + 1]; e += NT) {
+      const int u = P.am_row[e];
+      F[(u < ns ? u : u + din) + (size_t)j * m] += P.vals[P.am_src[e]];
+    }
+  }
+  __syncthreads();
+  if (P.diag)
+    for (int j = tid; j < ns; j += NT) F[(size_t)j * m + j] += P.diag[P.node_to_var[vp + j]];
+  __syncthreads();
+  if (mode == 2) {
+    cholesky_front<NT>(F, m, m, fs);
+    for (int j = tid; j < fs; j += NT) ptype[j] = 1;
+    if (tid == 0) {
+      P.npiv[p] = fs;
+      P.ndel[p] = 0;
+      P.cnt[3 * p] = P.cnt[3 * p + 1] = P.cnt[3 * p + 2] = 0;
+    }
+  } else {
+    const int nb = max(2, min(kNB, m));
+    int k = 0, nc = (mode == 1) ? m : fs;
+    while (k < nc) {
+      const int kstart = k;
+      int kw = 0;
+      panel_factor<NT>(F, m, m, vars, pre, ptype, W, m, nb, mode, P.ustab, P.ptol, k, nc,
+                       kw, sh);
+      trailing_update_wg<NT>(F, m, m, W, m, kstart, k, kw);
+    }
+    if (tid == 0) {
+      P.npiv[p] = nc;
+      P.ndel[p] = fs - nc;
+      P.cnt[3 * p] = sh.npos;
+      P.cnt[3 * p + 1] = sh.nneg;
+      P.cnt[3 * p + 2] = sh.n2;
+    }
+  }
+  __syncthreads();
+  double* Fg = P.F + P.foff[p];
+  for (int e = tid; e < m * m; e += NT) Fg[e] = F[e];
+}
+
+// --------------------------------------------------------------------------
+// Large fronts: one panel per launch on one workgroup (state in P.ps), the
+// trailing update spread over many workgroups (k_update).
+// --------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
+  const int p = nodes[blockIdx.x];
+  if (P.stat[p]) return;
+  __shared__ FrontShared<NT> sh;
+  __shared__ PanelState s_st;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_st = P.ps[p];
+    sh.npos = s_st.npos;
+    sh.nneg = s_st.nneg;
+    sh.n2 = s_st.n2;
+  }
+  __syncthreads();
+  int k = s_st.k, nc = s_st.nc, kw = 0;
+  const int kstart = k;
+  if (k < nc) {
+    const int m = P.m[p];
+    const int mode = P.mode[p];
+    const int nb = max(2, min(kNB, m));
+    panel_factor<NT>(P.F + P.foff[p], m, m, P.vars + P.ioff[p], P.pre + P.ioff[p],
+                 P.ptype + P.ioff[p], P.W + P.woff[p], P.cap[p], nb, mode, P.ustab,
+                 P.ptol, k, nc, kw, sh);
+    if (tid == 0 && k >= nc) {
+      const int fs = P.ns[p] + P.din[p];
+      P.npiv[p] = nc;
+      P.ndel[p] = fs - nc;
+      P.cnt[3 * p] = sh.npos;
+      P.cnt[3 * p + 1] = sh.nneg;
+      P.cnt[3 * p + 2] = sh.n2;
+    }
+  }
+  if (tid == 0) {
+    PanelState st = s_st;
+    st.k = k;
+    st.nc = nc;
+    st.kstart = kstart;
+    st.kw = kw;  // 0 when the front was already complete: no update
+    st.npos = sh.npos;
+    st.nneg = sh.nneg;
+    st.n2 = sh.n2;
+    P.ps[p] = st;
+  }
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// F[k:m, k:m] (lower) -= F[k:m, kstart:kstart+kw] * W[k:m, 0:kw]^T with
+// v_mfma_f64_16x16x4f64.  Workgroup = 64 x 64 tile of the region starting
+// at `base` (a lower bound of k); each wave a 32 x 32 quarter of it.
+// MFMA operand layout (checked on gfx950): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and receives D[l/16 + 4r][l%16], r = 0..3 (the FP64 16x16x4
+// output interleaves rows by 4, unlike the FP32 16x16x4 one).
+__global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
+                                                 const int* wtile, int base) {
+  const int p = wnode[blockIdx.x];
+  if (P.stat[p]) return;
+  const PanelState st = P.ps[p];
+  const int kw = st.kw;
+  if (kw == 0) return;
+  const int m = P.m[p], k = st.k, kstart = st.kstart;
+  const int t = wtile[blockIdx.x];
+  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (bi * (bi + 1) / 2 > t) bi--;
+  while ((bi + 1) * (bi + 2) / 2 <= t) bi++;
+  const int bj = t - bi * (bi + 1) / 2;
+  const int i0 = base + 64 * bi, j0 = base + 64 * bj;
+  if (i0 >= m || j0 >= m || i0 + 64 <= k || j0 + 64 <= k) return;
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int ri = i0 + 32 * (w >> 1), cj = j0 + 32 * (w & 1);
+  if (ri + 32 <= cj) return;  // strictly above the diagonal
+  double* F = P.F + P.foff[p];
+  const double* W = P.W + P.woff[p];
+  const int ldw = P.cap[p];
+  const int lr = l & 15, lk = l >> 4;
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  for (int kk = 0; kk < kw; kk += 4) {
+    const int c = kk + lk;
+    const bool ok = c < kw;
+    double av[2], bv[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const int row = ri + 16 * a + lr;
+      av[a] = (ok && row < m) ? F[row + (size_t)(kstart + c) * m] : 0.0;
+      const int col = cj + 16 * a + lr;
+      bv[a] = (ok && col < m) ? W[col + (size_t)c * ldw] : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = ri + 16 * a + lk + 4 * r;
+        const int j = cj + 16 * b + lr;
+        if (i < m && j < m && i >= j && j >= k) F[i + (size_t)j * m] -= acc[a][b][r];
+      }
 }
 
 // --------------------------------------------------------------------------
@@ -908,6 +1146,16 @@ struct Handle {
       d_asm_chunk, d_rowp, d_cols;
   DBuf<long long> d_invoff, d_foff, d_ioff, d_woff, d_uoff;
   DBuf<int> d_din, d_m, d_npiv, d_ndel, d_doff, d_stat, d_need, d_cnt;
+  DBuf<PanelState> d_ps;
+  // per level: small fronts (one workgroup each) and large fronts with
+  // their per-panel update work lists
+  std::vector<int> small_ptr, small_nodes, large_ptr, large_nodes, large_panels;
+  std::vector<int> lds_ptr, lds_nodes, lds_cap;  // per level: LDS fronts, max cap
+  int lds_front = kLdsFront;  // largest front dimension kept in LDS
+  DBuf<int> d_lds_nodes;
+  std::vector<int> upd_ptr, upd_node, upd_tile;  // upd_ptr per (level, panel)
+  std::vector<int> upd_lvl;                      // first upd_ptr slot of a level
+  DBuf<int> d_small_nodes, d_large_nodes, d_upd_node, d_upd_tile;
   DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_norm;
   DBuf<int> d_vars, d_pre, d_ptype;
   bool has_diag = false;
@@ -990,6 +1238,7 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   for (int l = 0; l < nlev; l++) {
     for (int q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; q++) {
       int k = H.lvl_nodes[q];
+      if (H.cap[k] <= H.lds_front) continue;  // assembled inside k_front_lds
       int nch = (H.cap[k] + kAsmCols - 1) / kAsmCols;
       for (int c = 0; c < nch; c++) {
         H.asm_node.push_back(k);
@@ -998,6 +1247,52 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     }
     H.asm_lvl_ptr[l + 1] = (int)H.asm_node.size();
   }
+  // Large fronts get the multi-workgroup panel/update path; their update
+  // tiles for panel t cover [t*(kNB-2), cap), since every panel but the last
+  // takes at least kNB-2 pivots.
+  H.small_ptr.assign(nlev + 1, 0);
+  H.large_ptr.assign(nlev + 1, 0);
+  H.lds_ptr.assign(nlev + 1, 0);
+  H.lds_cap.assign(nlev, 0);
+  H.small_nodes.clear();
+  H.large_nodes.clear();
+  H.lds_nodes.clear();
+  H.large_panels.assign(nlev, 0);
+  H.upd_ptr.assign(1, 0);
+  H.upd_lvl.assign(nlev + 1, 0);
+  H.upd_node.clear();
+  H.upd_tile.clear();
+  for (int l = 0; l < nlev; l++) {
+    for (int q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; q++) {
+      int k = H.lvl_nodes[q];
+      if (H.cap[k] <= H.lds_front) {
+        H.lds_nodes.push_back(k);
+        H.lds_cap[l] = std::max(H.lds_cap[l], H.cap[k]);
+      } else if (H.mode[k] != 2 && H.ns[k] + H.cs[k] > kLargeFront) {
+        H.large_nodes.push_back(k);
+        H.large_panels[l] = std::max(H.large_panels[l], H.cap[k] / (kNB - 2) + 1);
+      } else {
+        H.small_nodes.push_back(k);
+      }
+    }
+    H.lds_ptr[l + 1] = (int)H.lds_nodes.size();
+    H.small_ptr[l + 1] = (int)H.small_nodes.size();
+    H.large_ptr[l + 1] = (int)H.large_nodes.size();
+    H.upd_lvl[l] = (int)H.upd_ptr.size() - 1;
+    for (int t = 0; t < H.large_panels[l]; t++) {
+      const int base = t * (kNB - 2);
+      for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
+        const int k = H.large_nodes[q];
+        const int nt = (H.cap[k] - base + 63) / 64;
+        for (int tt = 0; tt < nt * (nt + 1) / 2; tt++) {
+          H.upd_node.push_back(k);
+          H.upd_tile.push_back(tt);
+        }
+      }
+      H.upd_ptr.push_back((int)H.upd_node.size());
+    }
+  }
+  H.upd_lvl[nlev] = (int)H.upd_ptr.size() - 1;
 }
 
 // Static maps that do not depend on the slack.
@@ -1095,6 +1390,12 @@ static void upload_static(Handle& H) {
   H.d_stat.alloc(N);
   H.d_need.alloc(N);
   H.d_cnt.alloc(3 * (size_t)N);
+  H.d_ps.alloc(N);
+  H.d_small_nodes.upload(H.small_nodes);
+  H.d_lds_nodes.upload(H.lds_nodes);
+  H.d_large_nodes.upload(H.large_nodes);
+  H.d_upd_node.upload(H.upd_node);
+  H.d_upd_tile.upload(H.upd_tile);
   H.d_F.alloc((size_t)H.f_total);
   H.d_vars.alloc((size_t)H.i_total);
   H.d_pre.alloc((size_t)H.i_total);
@@ -1132,6 +1433,7 @@ static DevPlan dev_plan(Handle& H) {
   P.stat = H.d_stat.p;
   P.need = H.d_need.p;
   P.cnt = H.d_cnt.p;
+  P.ps = H.d_ps.p;
   P.F = H.d_F.p;
   P.vars = H.d_vars.p;
   P.pre = H.d_pre.p;
@@ -1175,8 +1477,27 @@ static void run_factor_levels(Handle& H, hipStream_t st) {
     if (a1 > a0)
       hipLaunchKernelGGL(k_assemble, dim3(a1 - a0), dim3(kTPB), 0, st, P,
                          H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
-    hipLaunchKernelGGL(k_factor, dim3(n1 - n0), dim3(kTPB), 0, st, P,
-                       H.d_lvl_nodes.p + n0);
+    const int d0 = H.lds_ptr[l], d1 = H.lds_ptr[l + 1];
+    if (d1 > d0) {
+      const int c = H.lds_cap[l];
+      const size_t bytes = sizeof(double) * ((size_t)c * c + (size_t)c * kNB);
+      hipLaunchKernelGGL(k_front_lds<kTPB>, dim3(d1 - d0), dim3(kTPB), bytes, st, P,
+                         H.d_lds_nodes.p + d0);
+    }
+    const int s0 = H.small_ptr[l], s1 = H.small_ptr[l + 1];
+    if (s1 > s0)
+      hipLaunchKernelGGL(k_factor, dim3(s1 - s0), dim3(kTPB), 0, st, P,
+                         H.d_small_nodes.p + s0);
+    const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
+    for (int t = 0; t < H.large_panels[l]; t++) {
+      hipLaunchKernelGGL(k_panel<kPanelThreads>, dim3(g1 - g0), dim3(kPanelThreads), 0, st, P,
+                         H.d_large_nodes.p + g0);
+      const int slot = H.upd_lvl[l] + t;
+      const int u0 = H.upd_ptr[slot], u1 = H.upd_ptr[slot + 1];
+      if (u1 > u0)
+        hipLaunchKernelGGL(k_update, dim3(u1 - u0), dim3(kTPB), 0, st, P,
+                           H.d_upd_node.p + u0, H.d_upd_tile.p + u0, t * (kNB - 2));
+    }
   }
   HIPCHECK(hipGetLastError());
 }
@@ -1185,6 +1506,18 @@ static void run_factor_levels(Handle& H, hipStream_t st) {
 // symbolic analysis (create, get_perm, get_supernodes) needs no GPU.
 static void ensure_device(Handle& H) {
   if (H.uploaded) return;
+  // Fit the LDS front bound to this device (m^2 + 64 m doubles + ~4 KB).
+  int dev = 0, lim = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  HIPCHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  int mlds = kLdsFront;
+  while (mlds > 8 && (double)8 * ((double)mlds * mlds + (double)mlds * kNB) + 4096.0 > lim) mlds--;
+  if (mlds != H.lds_front) {
+    H.lds_front = mlds;
+    std::vector<int> slack(H.N);
+    for (int k = 0; k < H.N; k++) slack[k] = H.cap[k] - H.ns[k] - H.cs[k];
+    build_plan(H, slack);
+  }
   upload_static(H);
   HIPCHECK(hipEventCreate(&H.ev0));
   HIPCHECK(hipEventCreate(&H.ev1));
