@@ -68,6 +68,7 @@ constexpr int kWave = 64;
 constexpr int kAsmCols = 8;    // columns per assembly workgroup
 constexpr int kLargeFront = 256;  // fronts above this use k_panel + k_update
 constexpr int kPanelThreads = 1024;  // workgroup of the large-front panel
+constexpr int kNBLarge = 16;    // panel width of large fronts (narrow: the panel re-reads it per column)
 constexpr int kLdsFront = 96;   // default bound of fronts assembled and factored in LDS
 
 // --------------------------------------------------------------------------
@@ -100,6 +101,7 @@ struct DevPlan {
   const int* am_ptr;   // per own column (global index g in node_to_var order)
   const int* am_src;   // CSR index into vals
   const int* am_row;   // nominal row in the front
+  int lds_t;           // solve kernels: doubles of LDS before the staged block
   // dynamic per node
   int* din;            // delayed pivots received
   int* m;              // actual front dimension
@@ -124,6 +126,46 @@ struct DevPlan {
 };
 
 __device__ inline int unpiv(int v) { return v >= 0 ? v : -v - 1; }
+
+// v - sum_{c < n} a[c * sa] * b[c], accumulated in index order, with the
+// loads of a (global or LDS) issued in batches of 8 so that their latency
+// is paid once per batch rather than once per term.
+__device__ inline double sub_dot(double v, const double* a, size_t sa,
+                                 const double* b, int n) {
+  int c = 0;
+  for (; c + 8 <= n; c += 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = a[(size_t)(c + u) * sa];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v -= x[u] * b[c + u];
+  }
+  if (c < n) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = (c + u < n) ? a[(size_t)(c + u) * sa] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (c + u < n) v -= x[u] * b[c + u];
+  }
+  return v;
+}
+
+// sum_{i} a[i] * b[i] over i = i0, i0 + st, ... < i1, loads batched by 8.
+__device__ inline double strided_dot(const double* a, const double* b, int i0, int i1,
+                                     int st) {
+  double s = 0.0;
+  int i = i0;
+  for (; i + 7 * st < i1; i += 8 * st) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = a[i + u * st];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += x[u] * b[i + u * st];
+  }
+  for (; i < i1; i += st) s += a[i] * b[i];
+  return s;
+}
 
 // Row of child c's front (post-pivot) that holds the parent's pre-pivot
 // position i, or -1.  Children contribute their delayed pivots (rows
@@ -349,6 +391,7 @@ template <int NT>
 struct FrontShared {
   double wr[kNB];
   double wr2[kNB];
+  double wkk, wik, wii;  // W[k,kw], W[imax,kw], W1[imax] stashed by their owners
   RedBuf<NT> rb;
   int npos, nneg, n2;
 };
@@ -407,8 +450,9 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
     int lidx = 0x7fffffff;
     for (int i = k + tid; i < m; i += NT) {
       double v = F[i + (size_t)k * ld];
-      for (int c = 0; c < kw; c++) v -= F[i + (size_t)(kstart + c) * ld] * sh.wr[c];
+      v = sub_dot(v, F + i + (size_t)kstart * ld, ld, sh.wr, kw);
       W[i + (size_t)kw * ldw] = v;
+      if (i == k) sh.wkk = v;
       double a = fabs(v);
       if (i > k && i < nc) {
         if (a > lbest) {
@@ -420,13 +464,13 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
       }
     }
     block_argmax2<NT>(lbest, lidx, lrest, sh.rb);
-    const double awkk = fabs(W[k + (size_t)kw * ldw]);
+    const double wkk = sh.wkk;
+    const double awkk = fabs(wkk);
     int imax = 0;
     double awik = 0.0;
     if (k + 1 < nc) {
       imax = lidx;
-      awik = W[imax + (size_t)kw * ldw];
-      awik = fabs(awik);
+      awik = lbest;  // |W[imax, kw]|
     }
     const double colmax = fmax(awik, lrest);
     int step = 0, kpiv = -1;
@@ -459,15 +503,17 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
       double lrow = 0.0;
       for (int i = k + tid; i < m; i += NT) {
         double v = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
-        for (int c = 0; c < kw; c++) v -= F[i + (size_t)(kstart + c) * ld] * sh.wr2[c];
+        v = sub_dot(v, F + i + (size_t)kstart * ld, ld, sh.wr2, kw);
         W1[i] = v;
+        if (i == imax) sh.wii = v;
+        if (i == imax) sh.wik = W[imax + (size_t)kw * ldw];
         if (i != imax) lrow = fmax(lrow, fabs(v));
       }
       double dummy = -1.0;
       int di = 0;
       block_argmax2<NT>(dummy, di, lrow, sh.rb);
       const double rowmax = lrow;
-      const double awii = fabs(W1[imax]);
+      const double awii = fabs(sh.wii);
       if (mode == 1) {
         if (awkk >= alpha * colmax * (colmax / rowmax)) {
           step = 1;
@@ -484,8 +530,8 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
           step = 1;
           kpiv = imax;
         } else {
-          double wkk = W[k + (size_t)kw * ldw], wik = W[imax + (size_t)kw * ldw];
-          double det = fabs(wkk * W1[imax] - wik * wik);
+          const double wik = sh.wik;
+          double det = fabs(wkk * sh.wii - wik * wik);
           if (det >= ptol) {
             double delta = det / fmax(awii + awik, awkk + awik);
             if (delta >= ustab * colmax && delta >= ustab * rowmax) {
@@ -524,7 +570,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
       __syncthreads();
     }
     if (step == 1) {
-      const double d = W[k + (size_t)kw * ldw];
+      const double d = (kpiv == k) ? wkk : sh.wii;
       const double dinv = (zero_col || d == 0.0) ? 0.0 : 1.0 / d;
       for (int i = k + tid; i < m; i += NT) {
         double v = W[i + (size_t)kw * ldw];
@@ -536,9 +582,9 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
         else sh.nneg++;
       }
     } else {
-      const double d11 = W[k + (size_t)kw * ldw];
-      const double d21 = W[k + 1 + (size_t)kw * ldw];
-      const double d22 = W[k + 1 + (size_t)(kw + 1) * ldw];
+      const double d11 = wkk;       // W[k, kw]
+      const double d21 = sh.wik;    // W[imax, kw], now in row k+1
+      const double d22 = sh.wii;    // W1[imax], now W[k+1, kw+1]
       const double det = d11 * d22 - d21 * d21;
       const double detinv = 1.0 / det;
       const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
@@ -593,19 +639,25 @@ __device__ void trailing_update_wg(double* F, int ld, int m, const double* W,
     for (int a = 0; a < 4; a++)
 #pragma unroll
       for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
-    for (int c = 0; c < kw; c++) {
-      const double* Lc = F + (size_t)(kstart + c) * ld;
-      const double* Wc = W + (size_t)c * ldw;
-      double l[4], w[4];
+    for (int c0 = 0; c0 < kw; c0 += 4) {
+      double l[4][4], w[4][4];  // [column][row]: 32 loads issued together
 #pragma unroll
-      for (int a = 0; a < 4; a++) {
-        l[a] = (i0 + a < m) ? Lc[i0 + a] : 0.0;
-        w[a] = (j0 + a < m) ? Wc[j0 + a] : 0.0;
+      for (int cc = 0; cc < 4; cc++) {
+        const bool okc = c0 + cc < kw;
+        const double* Lc = F + (size_t)(kstart + c0 + cc) * ld;
+        const double* Wc = W + (size_t)(c0 + cc) * ldw;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          l[cc][a] = (okc && i0 + a < m) ? Lc[i0 + a] : 0.0;
+          w[cc][a] = (okc && j0 + a < m) ? Wc[j0 + a] : 0.0;
+        }
       }
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int cc = 0; cc < 4; cc++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] += l[a] * w[b];
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++) acc[a][b] += l[cc][a] * w[cc][b];
     }
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -818,7 +870,7 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
   if (k < nc) {
     const int m = P.m[p];
     const int mode = P.mode[p];
-    const int nb = max(2, min(kNB, m));
+    const int nb = max(2, min(kNBLarge, m));
     panel_factor<NT>(P.F + P.foff[p], m, m, P.vars + P.ioff[p], P.pre + P.ioff[p],
                  P.ptype + P.ioff[p], P.W + P.woff[p], P.cap[p], nb, mode, P.ustab,
                  P.ptol, k, nc, kw, sh);
@@ -874,28 +926,9 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
   const double* W = P.W + P.woff[p];
   const int ldw = P.cap[p];
   const int lr = l & 15, lk = l >> 4;
+  // Accumulator starts from the F tile; the A operand is -L, so the MFMA
+  // chain produces F - L W^T directly.  All loads are issued up front.
   dbl4 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
-  for (int kk = 0; kk < kw; kk += 4) {
-    const int c = kk + lk;
-    const bool ok = c < kw;
-    double av[2], bv[2];
-#pragma unroll
-    for (int a = 0; a < 2; a++) {
-      const int row = ri + 16 * a + lr;
-      av[a] = (ok && row < m) ? F[row + (size_t)(kstart + c) * m] : 0.0;
-      const int col = cj + 16 * a + lr;
-      bv[a] = (ok && col < m) ? W[col + (size_t)c * ldw] : 0.0;
-    }
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++)
-        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
-  }
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -904,20 +937,65 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
       for (int r = 0; r < 4; r++) {
         const int i = ri + 16 * a + lk + 4 * r;
         const int j = cj + 16 * b + lr;
-        if (i < m && j < m && i >= j && j >= k) F[i + (size_t)j * m] -= acc[a][b][r];
+        acc[a][b][r] = (i < m && j < m) ? F[i + (size_t)j * m] : 0.0;
+      }
+  constexpr int KS = kNBLarge / 4;  // k-steps of 4 (kw <= kNBLarge)
+  double av[KS][2], bv[KS][2];
+#pragma unroll
+  for (int q = 0; q < KS; q++) {
+    const int c = 4 * q + lk;
+    const bool ok = c < kw;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const int row = ri + 16 * a + lr;
+      av[q][a] = (ok && row < m) ? -F[row + (size_t)(kstart + c) * m] : 0.0;
+      const int col = cj + 16 * a + lr;
+      bv[q][a] = (ok && col < m) ? W[col + (size_t)c * ldw] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < KS; q++)
+    if (4 * q < kw) {
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q][a], bv[q][b], acc[a][b], 0, 0, 0);
+    }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = ri + 16 * a + lk + 4 * r;
+        const int j = cj + 16 * b + lr;
+        if (i < m && j < m && i >= j && j >= k) F[i + (size_t)j * m] = acc[a][b][r];
       }
 }
 
 // --------------------------------------------------------------------------
 // Solve.  Forward: front vector in LDS (post-pivot order), pulled from the
-// right-hand side and the children's update vectors.
+// right-hand side and the children's update vectors; the unit-lower L11 is
+// applied in blocks of kSB columns: the diagonal block by one wave with the
+// block held in registers, the rows below by the whole workgroup.
 // --------------------------------------------------------------------------
+constexpr int kSB = 64;
+
+// End of the solve block starting at jb: kSB columns, shortened so that a
+// 2x2 pivot pair is never split.
+__device__ inline int block_end(const int* ptype, int jb, int npiv, int chol) {
+  int je = min(npiv, jb + kSB);
+  if (!chol && je < npiv && ptype[je - 1] == 2) je--;
+  return je;
+}
+
 __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
                                                   double* x, int ldx, int nrhs,
                                                   long long ustride, int chol) {
   extern __shared__ double t[];
   const int p = nodes[blockIdx.x];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const int m = P.m[p], npiv = P.npiv[p];
   const int ns = P.ns[p], din = P.din[p];
   const int fs = ns + din;
@@ -941,28 +1019,40 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
       t[i] = v;
     }
     __syncthreads();
-    if (chol) {
-      for (int j = 0; j < npiv; j++) {
-        const double tj = t[j] / L[j + (size_t)j * m];
-        for (int i = j + 1 + tid; i < m; i += kTPB) t[i] -= L[i + (size_t)j * m] * tj;
-        __syncthreads();
-        if (tid == 0) t[j] = tj;
-        __syncthreads();
+    for (int jb = 0; jb < npiv;) {
+      const int je = block_end(ptype, jb, npiv, chol);
+      const int bs = je - jb;
+      double* Lb = t + P.lds_t;  // kSB x kSB block, column major
+      for (int e = tid; e < bs * bs; e += kTPB) {
+        const int c = e / bs, rr = e - c * bs;
+        Lb[rr + c * kSB] = L[jb + rr + (size_t)(jb + c) * m];
       }
-    } else {
-      for (int j = 0; j < npiv;) {
-        if (ptype[j] == 1) {
-          const double tj = t[j];
-          for (int i = j + 1 + tid; i < m; i += kTPB) t[i] -= L[i + (size_t)j * m] * tj;
-          j += 1;
-        } else {
-          const double a = t[j], b = t[j + 1];
-          for (int i = j + 2 + tid; i < m; i += kTPB)
-            t[i] -= L[i + (size_t)j * m] * a + L[i + (size_t)(j + 1) * m] * b;
-          j += 2;
+      __syncthreads();
+      if (w == 0) {
+        const int row = jb + lane;
+        double tr = lane < bs ? t[row] : 0.0;
+        const double dr = (chol && lane < bs) ? Lb[lane + lane * kSB] : 1.0;
+        for (int c = 0; c < bs; c++) {
+          const int ty = chol ? 1 : ptype[jb + c];
+          if (ty == 1) {
+            if (chol && lane == c) tr = tr / dr;
+            const double tc = __shfl(tr, c);
+            if (lane > c && lane < bs) tr -= Lb[lane + c * kSB] * tc;
+          } else if (ty == 2) {
+            const double ta = __shfl(tr, c), tb = __shfl(tr, c + 1);
+            if (lane > c + 1 && lane < bs)
+              tr -= Lb[lane + c * kSB] * ta + Lb[lane + (c + 1) * kSB] * tb;
+          }
         }
-        __syncthreads();
+        if (lane < bs) t[row] = tr;
       }
+      __syncthreads();
+      for (int i = je + tid; i < m; i += kTPB) {
+        double s = 0.0;
+        t[i] = sub_dot(t[i], L + i + (size_t)jb * m, m, t + jb, je - jb);
+      }
+      __syncthreads();
+      jb = je;
     }
     double* U = P.U + P.uoff[p] + (long long)r * ustride;
     for (int i = npiv + tid; i < m; i += kTPB) U[i - npiv] = t[i];
@@ -988,7 +1078,9 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
   }
 }
 
-// Backward: t1 = x[pivots] - L21^T x[rows below], then L11^T back solve.
+// Backward: t1 = x[pivots] - L21^T x[rows below], then the L11^T back solve
+// in blocks from the bottom (dot products over the rows below a block by
+// the whole workgroup, the diagonal block by one wave in registers).
 __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
                                                    double* x, int ldx, int nrhs,
                                                    int chol) {
@@ -1012,46 +1104,54 @@ __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
     // t[j] -= sum_{i >= npiv} L[i,j] t[i]  (one wave per column)
     for (int j = w; j < npiv; j += kTPB / kWave) {
       double s = 0.0;
-      for (int i = npiv + lane; i < m; i += kWave) s += L[i + (size_t)j * m] * t[i];
+      s = strided_dot(L + (size_t)j * m, t, npiv + lane, m, kWave);
       for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
       if (lane == 0) t[j] -= s;
     }
     __syncthreads();
-    // L11^T back substitution (wave 0)
-    if (w == 0) {
-      for (int j = npiv - 1; j >= 0;) {
-        if (ptype[j] == 3) {  // 2x2 block (j-1, j)
-          double s0 = 0.0, s1 = 0.0;
-          for (int i = j + 1 + lane; i < npiv; i += kWave) {
-            s0 += L[i + (size_t)(j - 1) * m] * t[i];
-            s1 += L[i + (size_t)j * m] * t[i];
-          }
-          for (int off = kWave / 2; off > 0; off >>= 1) {
-            s0 += __shfl_xor(s0, off);
-            s1 += __shfl_xor(s1, off);
-          }
-          if (lane == 0) {
-            t[j - 1] -= s0;
-            t[j] -= s1;
-          }
-          __builtin_amdgcn_s_waitcnt(0);
-          __builtin_amdgcn_wave_barrier();
-          j -= 2;
-        } else {
+    for (int je = npiv; je > 0;) {
+      int jb = max(0, je - kSB);
+      if (!chol && jb > 0 && ptype[jb] == 3) jb++;
+      const int bs = je - jb;
+      // rows of L11 below the block are final: dot products per column
+      if (je < npiv) {
+        for (int j = jb + w; j < je; j += kTPB / kWave) {
           double s = 0.0;
-          for (int i = j + 1 + lane; i < npiv; i += kWave) s += L[i + (size_t)j * m] * t[i];
+          s = strided_dot(L + (size_t)j * m, t, je + lane, npiv, kWave);
           for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-          if (lane == 0) {
-            double v = t[j] - s;
-            t[j] = chol ? v / L[j + (size_t)j * m] : v;
-          }
-          __builtin_amdgcn_s_waitcnt(0);
-          __builtin_amdgcn_wave_barrier();
-          j -= 1;
+          if (lane == 0) t[j] -= s;
         }
+        __syncthreads();
       }
+      double* Lb = t + P.lds_t;
+      for (int e = tid; e < bs * bs; e += kTPB) {
+        const int c = e / bs, rr = e - c * bs;
+        Lb[rr + c * kSB] = L[jb + rr + (size_t)(jb + c) * m];
+      }
+      __syncthreads();
+      if (w == 0) {
+        // lane c owns column jb + c of the block
+        const int col = jb + lane;
+        const double dc = (chol && lane < bs) ? Lb[lane + lane * kSB] : 1.0;
+        double tv = lane < bs ? t[col] : 0.0;
+        double acc = 0.0;
+        for (int i = bs - 1; i >= 0; i--) {
+          const int ty = chol ? 1 : ptype[jb + i];
+          if (ty == 1) {
+            if (lane == i) tv = chol ? (tv - acc) / dc : tv - acc;
+            const double ti = __shfl(tv, i);
+            if (lane < i) acc += Lb[i + lane * kSB] * ti;
+          } else if (ty == 3) {
+            if (lane == i || lane == i - 1) tv -= acc;
+            const double ti = __shfl(tv, i), tj = __shfl(tv, i - 1);
+            if (lane < i - 1) acc += Lb[i + lane * kSB] * ti + Lb[i - 1 + lane * kSB] * tj;
+          }
+        }
+        if (lane < bs) t[col] = tv;
+      }
+      __syncthreads();
+      je = jb;
     }
-    __syncthreads();
     for (int j = tid; j < npiv; j += kTPB) xr[unpiv(vars[j])] = t[j];
     __syncthreads();
   }
@@ -1270,7 +1370,7 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
         H.lds_cap[l] = std::max(H.lds_cap[l], H.cap[k]);
       } else if (H.mode[k] != 2 && H.ns[k] + H.cs[k] > kLargeFront) {
         H.large_nodes.push_back(k);
-        H.large_panels[l] = std::max(H.large_panels[l], H.cap[k] / (kNB - 2) + 1);
+        H.large_panels[l] = std::max(H.large_panels[l], (H.cap[k] - H.cs[k]) / (kNBLarge - 2) + 1);
       } else {
         H.small_nodes.push_back(k);
       }
@@ -1280,7 +1380,7 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     H.large_ptr[l + 1] = (int)H.large_nodes.size();
     H.upd_lvl[l] = (int)H.upd_ptr.size() - 1;
     for (int t = 0; t < H.large_panels[l]; t++) {
-      const int base = t * (kNB - 2);
+      const int base = t * (kNBLarge - 2);
       for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
         const int k = H.large_nodes[q];
         const int nt = (H.cap[k] - base + 63) / 64;
@@ -1496,7 +1596,7 @@ static void run_factor_levels(Handle& H, hipStream_t st) {
       const int u0 = H.upd_ptr[slot], u1 = H.upd_ptr[slot + 1];
       if (u1 > u0)
         hipLaunchKernelGGL(k_update, dim3(u1 - u0), dim3(kTPB), 0, st, P,
-                           H.d_upd_node.p + u0, H.d_upd_tile.p + u0, t * (kNB - 2));
+                           H.d_upd_node.p + u0, H.d_upd_tile.p + u0, t * (kNBLarge - 2));
     }
   }
   HIPCHECK(hipGetLastError());
@@ -1600,7 +1700,9 @@ static void run_solve_levels(Handle& H, double* dx, int ldx, int nrhs, hipStream
   DevPlan P = dev_plan(H);
   const int nlev = (int)H.lvl_ptr.size() - 1;
   const int chol = H.opt.solver_type == KKT_LDL_CHOLESKY;
-  size_t lds = sizeof(double) * (size_t)H.max_cap;
+  const int lds_t = (H.max_cap + 31) & ~31;
+  P.lds_t = lds_t;
+  size_t lds = sizeof(double) * ((size_t)lds_t + (size_t)kSB * kSB);
   if (lds > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
   for (int l = 0; l < nlev; l++) {
     int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
