@@ -1227,8 +1227,19 @@ struct DBuf {
   }
 };
 
+enum KernelClass {
+  KC_PREPARE, KC_ASSEMBLE, KC_FRONT_LDS, KC_FACTOR, KC_PANEL, KC_UPDATE,
+  KC_FORWARD, KC_BACKWARD, KC_REFINE, KC_COUNT
+};
+
 struct Handle {
   kkt_ldl_options opt;
+  // optional per-kernel-class device timing (events around every launch)
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_ev;   // pairs
+  std::vector<int> prof_cls;
+  double prof_ms[KC_COUNT] = {0};
+  int prof_n[KC_COUNT] = {0};
   int n = 0, nnz = 0;
   std::vector<int> rowp, cols;
   SymbolicResult S;
@@ -1566,37 +1577,96 @@ static std::vector<int> initial_slack(const Handle& H) {
   return slack;
 }
 
+// Record an event pair around a launch when profiling is on.
+struct ProfScope {
+  Handle& H;
+  hipStream_t st;
+  bool on;
+  size_t slot = 0;
+  ProfScope(Handle& h, hipStream_t s, int cls) : H(h), st(s), on(h.profiling) {
+    if (!on) return;
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    H.prof_ev.push_back(a);
+    H.prof_ev.push_back(b);
+    H.prof_cls.push_back(cls);
+    slot = H.prof_ev.size() - 2;
+    HIPCHECK(hipEventRecord(a, st));
+  }
+  ~ProfScope() {
+    if (on) (void)hipEventRecord(H.prof_ev[slot + 1], st);
+  }
+};
+
+static void prof_reset(Handle& H) {
+  for (auto e : H.prof_ev) (void)hipEventDestroy(e);
+  H.prof_ev.clear();
+  H.prof_cls.clear();
+  for (int c = 0; c < KC_COUNT; c++) {
+    H.prof_ms[c] = 0.0;
+    H.prof_n[c] = 0;
+  }
+}
+
+static void prof_collect(Handle& H) {
+  if (!H.profiling) return;
+  for (size_t q = 0; q < H.prof_cls.size(); q++) {
+    float ms = 0.f;
+    HIPCHECK(hipEventSynchronize(H.prof_ev[2 * q + 1]));
+    HIPCHECK(hipEventElapsedTime(&ms, H.prof_ev[2 * q], H.prof_ev[2 * q + 1]));
+    H.prof_ms[H.prof_cls[q]] += ms;
+    H.prof_n[H.prof_cls[q]]++;
+  }
+  for (auto e : H.prof_ev) (void)hipEventDestroy(e);
+  H.prof_ev.clear();
+  H.prof_cls.clear();
+}
+
 static void run_factor_levels(Handle& H, hipStream_t st) {
   DevPlan P = dev_plan(H);
   const int nlev = (int)H.lvl_ptr.size() - 1;
   for (int l = 0; l < nlev; l++) {
     int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
-    hipLaunchKernelGGL(k_prepare, dim3((n1 - n0 + kTPB - 1) / kTPB), dim3(kTPB), 0, st,
-                       P, H.d_lvl_nodes.p + n0, n1 - n0);
+    {
+      ProfScope ps_(H, st, KC_PREPARE);
+      hipLaunchKernelGGL(k_prepare, dim3((n1 - n0 + kTPB - 1) / kTPB), dim3(kTPB), 0, st,
+                         P, H.d_lvl_nodes.p + n0, n1 - n0);
+    }
     int a0 = H.asm_lvl_ptr[l], a1 = H.asm_lvl_ptr[l + 1];
-    if (a1 > a0)
+    if (a1 > a0) {
+      ProfScope ps_(H, st, KC_ASSEMBLE);
       hipLaunchKernelGGL(k_assemble, dim3(a1 - a0), dim3(kTPB), 0, st, P,
                          H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
+    }
     const int d0 = H.lds_ptr[l], d1 = H.lds_ptr[l + 1];
     if (d1 > d0) {
       const int c = H.lds_cap[l];
       const size_t bytes = sizeof(double) * ((size_t)c * c + (size_t)c * kNB);
+      ProfScope ps_(H, st, KC_FRONT_LDS);
       hipLaunchKernelGGL(k_front_lds<kTPB>, dim3(d1 - d0), dim3(kTPB), bytes, st, P,
                          H.d_lds_nodes.p + d0);
     }
     const int s0 = H.small_ptr[l], s1 = H.small_ptr[l + 1];
-    if (s1 > s0)
+    if (s1 > s0) {
+      ProfScope ps_(H, st, KC_FACTOR);
       hipLaunchKernelGGL(k_factor, dim3(s1 - s0), dim3(kTPB), 0, st, P,
                          H.d_small_nodes.p + s0);
+    }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     for (int t = 0; t < H.large_panels[l]; t++) {
-      hipLaunchKernelGGL(k_panel<kPanelThreads>, dim3(g1 - g0), dim3(kPanelThreads), 0, st, P,
-                         H.d_large_nodes.p + g0);
+      {
+        ProfScope ps_(H, st, KC_PANEL);
+        hipLaunchKernelGGL(k_panel<kPanelThreads>, dim3(g1 - g0), dim3(kPanelThreads), 0, st,
+                           P, H.d_large_nodes.p + g0);
+      }
       const int slot = H.upd_lvl[l] + t;
       const int u0 = H.upd_ptr[slot], u1 = H.upd_ptr[slot + 1];
-      if (u1 > u0)
+      if (u1 > u0) {
+        ProfScope ps_(H, st, KC_UPDATE);
         hipLaunchKernelGGL(k_update, dim3(u1 - u0), dim3(kTPB), 0, st, P,
                            H.d_upd_node.p + u0, H.d_upd_tile.p + u0, t * (kNBLarge - 2));
+      }
     }
   }
   HIPCHECK(hipGetLastError());
@@ -1639,7 +1709,9 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
     H.d_diag.alloc(H.n);
     HIPCHECK(hipMemcpyAsync(H.d_diag.p, diag, sizeof(double) * H.n, kind, st));
   }
+  prof_reset(H);
   for (int attempt = 0;; attempt++) {
+    if (attempt > 0) prof_reset(H);
     HIPCHECK(hipEventRecord(H.ev0, st));
     run_factor_levels(H, st);
     HIPCHECK(hipEventRecord(H.ev1, st));
@@ -1668,6 +1740,7 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
     H.stats.replans++;
   }
   HIPCHECK(hipEventElapsedTime(&H.last_factor_ms, H.ev0, H.ev1));
+  prof_collect(H);
   H.h_npiv.resize(H.N);
   H.h_ndel.resize(H.N);
   H.h_cnt.resize(3 * (size_t)H.N);
@@ -1706,11 +1779,13 @@ static void run_solve_levels(Handle& H, double* dx, int ldx, int nrhs, hipStream
   if (lds > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
   for (int l = 0; l < nlev; l++) {
     int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
+    ProfScope ps_(H, st, KC_FORWARD);
     hipLaunchKernelGGL(k_forward, dim3(n1 - n0), dim3(kTPB), lds, st, P,
                        H.d_lvl_nodes.p + n0, dx, ldx, nrhs, (long long)H.u_total, chol);
   }
   for (int l = nlev - 1; l >= 0; l--) {
     int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
+    ProfScope ps_(H, st, KC_BACKWARD);
     hipLaunchKernelGGL(k_backward, dim3(n1 - n0), dim3(kTPB), lds, st, P,
                        H.d_lvl_nodes.p + n0, dx, ldx, nrhs, chol);
   }
@@ -1732,6 +1807,7 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
     dx = H.d_x.p;
   }
   int status = 0;
+  prof_reset(H);
   HIPCHECK(hipEventRecord(H.ev2, st));
   if (refine) {
     H.d_b.alloc(tot);
@@ -1746,6 +1822,7 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
     double worst = 0.0;
     const dim3 rg((unsigned)((H.n * 16 + kTPB - 1) / kTPB));
     for (;; it++) {
+      ProfScope ps_(H, st, KC_REFINE);
       hipLaunchKernelGGL(k_residual, rg, dim3(kTPB), 0, st, H.n, H.d_rowp.p, H.d_cols.p,
                          H.d_vals.p, H.has_diag ? H.d_diag.p : nullptr, dx, H.d_b.p,
                          H.d_r.p, ldx, nrhs);
@@ -1773,6 +1850,7 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
                                                 : hipMemcpyDeviceToDevice, st));
   HIPCHECK(hipStreamSynchronize(st));
   HIPCHECK(hipEventElapsedTime(&H.last_solve_ms, H.ev2, H.ev3));
+  prof_collect(H);
   return status;
 }
 
@@ -1782,6 +1860,8 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   if (o) H.opt = *o;
   else kkt_ldl_default_options(&H.opt);
   if (n <= 0) throw Error(-1, "n must be positive");
+  if (H.opt.precision != KKT_LDL_FP64)
+    throw Error(-8, "only the FP64 factor is available in this build");
   H.n = n;
   H.nnz = rowp[n];
   H.rowp.assign(rowp, rowp + n + 1);
@@ -1999,6 +2079,25 @@ int kkt_ldl_last_times(kkt_ldl_t h, double* factor_ms, double* solve_ms) {
     if (factor_ms) *factor_ms = h->h.last_factor_ms;
     if (solve_ms) *solve_ms = h->h.last_solve_ms;
     return 0;
+  });
+}
+
+int kkt_ldl_set_profiling(kkt_ldl_t h, int on) {
+  return guarded([&] {
+    if (!h) throw Error(-1, "null argument");
+    h->h.profiling = on != 0;
+    return 0;
+  });
+}
+
+int kkt_ldl_get_kernel_times(kkt_ldl_t h, double* ms, int* launches, int nclass) {
+  return guarded([&] {
+    if (!h) throw Error(-1, "null argument");
+    for (int c = 0; c < nclass && c < kktldl::KC_COUNT; c++) {
+      if (ms) ms[c] = h->h.prof_ms[c];
+      if (launches) launches[c] = h->h.prof_n[c];
+    }
+    return (int)kktldl::KC_COUNT;
   });
 }
 

@@ -92,8 +92,12 @@ EXPORTS = (
     "kkt_ldl_solve", "kkt_ldl_solve_refined", "kkt_ldl_inertia",
     "kkt_ldl_get_perm", "kkt_ldl_get_sizes", "kkt_ldl_get_supernodes",
     "kkt_ldl_get_node_info", "kkt_ldl_get_stats", "kkt_ldl_last_times",
+    "kkt_ldl_set_profiling", "kkt_ldl_get_kernel_times",
     "kkt_ldl_last_error", "kkt_ldl_destroy",
 )
+
+KERNEL_CLASSES = ("prepare", "assemble", "front_lds", "factor_small", "panel",
+                  "update", "forward", "backward", "residual")
 
 _lib = None
 _i32p = ctypes.POINTER(ctypes.c_int)
@@ -122,6 +126,8 @@ def load_library(path=LIB_PATH):
     lib.kkt_ldl_get_node_info.argtypes = [_vp] + [_i32p] * 4
     lib.kkt_ldl_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     lib.kkt_ldl_last_times.argtypes = [_vp, _f64p, _f64p]
+    lib.kkt_ldl_set_profiling.argtypes = [_vp, ctypes.c_int]
+    lib.kkt_ldl_get_kernel_times.argtypes = [_vp, _f64p, _i32p, ctypes.c_int]
     lib.kkt_ldl_last_error.restype = ctypes.c_char_p
     lib.kkt_ldl_destroy.argtypes = [_vp]
     for name in EXPORTS:
@@ -380,6 +386,17 @@ class SparseLDL:
         s = Stats()
         _check(self.lib.kkt_ldl_get_stats(self.h, ctypes.byref(s)), "get_stats")
         return s.as_dict()
+
+    def set_profiling(self, on=True):
+        _check(self.lib.kkt_ldl_set_profiling(self.h, int(bool(on))), "set_profiling")
+
+    def kernel_times(self):
+        """{class: (device ms, launches)} of the last factor or solve."""
+        nc = len(KERNEL_CLASSES)
+        ms = (ctypes.c_double * nc)()
+        nl = (ctypes.c_int * nc)()
+        _check(self.lib.kkt_ldl_get_kernel_times(self.h, ms, nl, nc), "get_kernel_times")
+        return {k: (ms[i], nl[i]) for i, k in enumerate(KERNEL_CLASSES)}
 
     def last_times(self):
         f, s = ctypes.c_double(), ctypes.c_double()

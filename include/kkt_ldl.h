@@ -123,6 +123,16 @@ int kkt_ldl_get_stats(kkt_ldl_t h, kkt_ldl_stats* st);
  * (hipEvent based). */
 int kkt_ldl_last_times(kkt_ldl_t h, double* factor_ms, double* solve_ms);
 
+/* Per-kernel-class device timing: when on, every launch of the next
+ * factor / solve is bracketed by hipEvents on the caller's stream (this adds
+ * launch latency; use it for attribution, not for the headline time).
+ * Classes: 0 prepare, 1 assembly, 2 LDS fronts, 3 small fronts, 4 panel,
+ * 5 MFMA trailing update, 6 forward solve, 7 backward solve, 8 residual.
+ * No counterpart in the reference, which has no kernel instrumentation
+ * (amigo/algorithm/iteration_logger.py:28-39 only timestamps iterations). */
+int kkt_ldl_set_profiling(kkt_ldl_t h, int on);
+int kkt_ldl_get_kernel_times(kkt_ldl_t h, double* ms, int* launches, int nclass);
+
 const char* kkt_ldl_last_error(void);
 
 void kkt_ldl_destroy(kkt_ldl_t h);
