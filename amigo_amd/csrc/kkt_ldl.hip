@@ -113,10 +113,12 @@ struct DevPlan {
   int* cnt;            // [3N]: npos, nneg, n2x2
   PanelState* ps;      // [N]
   // arenas
-  double* F;
+  void* F;             // fronts, element type T (double or float)
   int* vars;
   int* pre;
   int* ptype;          // per front position: 1 = 1x1, 2/3 = first/second of 2x2
+  double* dv;          // D in FP64 per front position: d11 / d22 (and 1x1 d)
+  double* dv2;         // FP64 off-diagonal d21 of a 2x2 block, at its first position
   double* W;
   double* U;
   // numeric input
@@ -130,20 +132,21 @@ __device__ inline int unpiv(int v) { return v >= 0 ? v : -v - 1; }
 // v - sum_{c < n} a[c * sa] * b[c], accumulated in index order, with the
 // loads of a (global or LDS) issued in batches of 8 so that their latency
 // is paid once per batch rather than once per term.
-__device__ inline double sub_dot(double v, const double* a, size_t sa,
+template <class TA>
+__device__ inline double sub_dot(double v, const TA* a, size_t sa,
                                  const double* b, int n) {
   int c = 0;
   for (; c + 8 <= n; c += 8) {
     double x[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) x[u] = a[(size_t)(c + u) * sa];
+    for (int u = 0; u < 8; u++) x[u] = (double)a[(size_t)(c + u) * sa];
 #pragma unroll
     for (int u = 0; u < 8; u++) v -= x[u] * b[c + u];
   }
   if (c < n) {
     double x[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) x[u] = (c + u < n) ? a[(size_t)(c + u) * sa] : 0.0;
+    for (int u = 0; u < 8; u++) x[u] = (c + u < n) ? (double)a[(size_t)(c + u) * sa] : 0.0;
 #pragma unroll
     for (int u = 0; u < 8; u++)
       if (c + u < n) v -= x[u] * b[c + u];
@@ -152,33 +155,41 @@ __device__ inline double sub_dot(double v, const double* a, size_t sa,
 }
 
 // sum_{i} a[i] * b[i] over i = i0, i0 + st, ... < i1, loads batched by 8.
-__device__ inline double strided_dot(const double* a, const double* b, int i0, int i1,
+template <class TA>
+__device__ inline double strided_dot(const TA* a, const double* b, int i0, int i1,
                                      int st) {
   double s = 0.0;
   int i = i0;
   for (; i + 7 * st < i1; i += 8 * st) {
     double x[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) x[u] = a[i + u * st];
+    for (int u = 0; u < 8; u++) x[u] = (double)a[i + u * st];
 #pragma unroll
     for (int u = 0; u < 8; u++) s += x[u] * b[i + u * st];
   }
-  for (; i < i1; i += st) s += a[i] * b[i];
+  for (; i < i1; i += st) s += (double)a[i] * b[i];
   return s;
 }
 
 // Row of child c's front (post-pivot) that holds the parent's pre-pivot
 // position i, or -1.  Children contribute their delayed pivots (rows
 // npiv_c .. fs_c-1) and their nominal contribution rows (fs_c + q).
+template <class T>
 struct ChildView {
-  const double* F;
+  const T* F;
   int m, fs, npiv, ndel, doff;
   long long inv;
 };
 
-__device__ inline ChildView load_child(const DevPlan& P, int c) {
-  ChildView v;
-  v.F = P.F + P.foff[c];
+template <class T>
+__device__ inline T* front_ptr(const DevPlan& P, int p) {
+  return (T*)P.F + P.foff[p];
+}
+
+template <class T>
+__device__ inline ChildView<T> load_child(const DevPlan& P, int c) {
+  ChildView<T> v;
+  v.F = front_ptr<T>(P, c);
   v.m = P.m[c];
   v.fs = P.ns[c] + P.din[c];
   v.npiv = P.npiv[c];
@@ -188,8 +199,8 @@ __device__ inline ChildView load_child(const DevPlan& P, int c) {
   return v;
 }
 
-__device__ inline int child_row(const DevPlan& P, const ChildView& cv, int i,
-                                int ns, int din) {
+template <class CV>
+__device__ inline int child_row(const DevPlan& P, const CV& cv, int i, int ns, int din) {
   if (i < ns) {
     int q = P.invnom[cv.inv + i];
     return q >= 0 ? cv.fs + q : -1;
@@ -245,6 +256,7 @@ __global__ void k_prepare(DevPlan P, const int* nodes, int count) {
 // --------------------------------------------------------------------------
 constexpr int kChildCache = 64;
 
+template <class T>
 __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
                                                    const int* wchunk) {
   const int p = wnode[blockIdx.x];
@@ -256,12 +268,12 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
   const int j0 = chunk * kAsmCols;
   if (j0 >= m) return;
   const int j1 = min(m, j0 + kAsmCols);
-  __shared__ ChildView s_cv[kChildCache];
+  __shared__ ChildView<T> s_cv[kChildCache];
   __shared__ int s_list[kChildCache];
   __shared__ int s_rj[kChildCache];
   __shared__ int s_nl;
   const int ncache = min(nch, kChildCache);
-  for (int t = tid; t < ncache; t += kTPB) s_cv[t] = load_child(P, P.ch[c0 + t]);
+  for (int t = tid; t < ncache; t += kTPB) s_cv[t] = load_child<T>(P, P.ch[c0 + t]);
   int* vars = P.vars + P.ioff[p];
   int* pre = P.pre + P.ioff[p];
   if (chunk == 0) {
@@ -277,10 +289,10 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
     }
   }
   __syncthreads();
-  double* F = P.F + P.foff[p];
+  T* F = front_ptr<T>(P, p);
   const bool cached = nch <= kChildCache;
   for (int j = j0; j < j1; j++) {
-    double* Fj = F + (size_t)j * m;
+    T* Fj = F + (size_t)j * m;
     if (cached) {
       // children that own position j, compacted in child order (wave 0)
       if (tid < kWave) {
@@ -298,27 +310,27 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
       for (int i = j + tid; i < m; i += kTPB) {
         double v = 0.0;
         for (int t = 0; t < nl; t++) {
-          const ChildView& cv = s_cv[s_list[t]];
+          const ChildView<T>& cv = s_cv[s_list[t]];
           const int ri = child_row(P, cv, i, ns, din);
           if (ri >= 0) {
             const int rj = s_rj[t];
-            v += cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
+            v += (double)cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
           }
         }
-        Fj[i] = v;
+        Fj[i] = (T)v;
       }
       __syncthreads();
     } else {
       for (int i = j + tid; i < m; i += kTPB) {
         double v = 0.0;
         for (int q = 0; q < nch; q++) {
-          ChildView cv = load_child(P, P.ch[c0 + q]);
+          ChildView<T> cv = load_child<T>(P, P.ch[c0 + q]);
           const int rj = child_row(P, cv, j, ns, din);
           if (rj < 0) continue;
           const int ri = child_row(P, cv, i, ns, din);
-          if (ri >= 0) v += cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
+          if (ri >= 0) v += (double)cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
         }
-        Fj[i] = v;
+        Fj[i] = (T)v;
       }
     }
   }
@@ -327,16 +339,17 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
   const int vp = P.vptr[p];
   for (int j = j0; j < min(j1, ns); j++) {
     const int g = vp + j;
-    double* Fj = F + (size_t)j * m;
+    T* Fj = F + (size_t)j * m;
     for (int e = P.am_ptr[g] + tid; e < P.am_ptr[g + 1]; e += kTPB) {
       const int u = P.am_row[e];
-      Fj[u < ns ? u : u + din] += P.vals[P.am_src[e]];
+      const int r = u < ns ? u : u + din;
+      Fj[r] = (T)((double)Fj[r] + P.vals[P.am_src[e]]);
     }
   }
   __syncthreads();
   if (P.diag)
     for (int j = j0 + tid; j < min(j1, ns); j += kTPB)
-      F[(size_t)j * m + j] += P.diag[P.node_to_var[vp + j]];
+      F[(size_t)j * m + j] = (T)((double)F[(size_t)j * m + j] + P.diag[P.node_to_var[vp + j]]);
 }
 
 // --------------------------------------------------------------------------
@@ -396,8 +409,8 @@ struct FrontShared {
   int npos, nneg, n2;
 };
 
-template <int NT>
-__device__ inline void sym_swap(double* F, int ld, int m, int* vars, int* pre,
+template <int NT, class T>
+__device__ inline void sym_swap(T* F, int ld, int m, int* vars, int* pre,
                                 int a, int b) {
   const int tid = threadIdx.x;
   if (tid == 0) {
@@ -407,22 +420,22 @@ __device__ inline void sym_swap(double* F, int ld, int m, int* vars, int* pre,
     t = pre[a];
     pre[a] = pre[b];
     pre[b] = t;
-    double d = F[a + (size_t)a * ld];
+    T d = F[a + (size_t)a * ld];
     F[a + (size_t)a * ld] = F[b + (size_t)b * ld];
     F[b + (size_t)b * ld] = d;
   }
   for (int j = tid; j < a; j += NT) {
-    double t = F[a + (size_t)j * ld];
+    T t = F[a + (size_t)j * ld];
     F[a + (size_t)j * ld] = F[b + (size_t)j * ld];
     F[b + (size_t)j * ld] = t;
   }
   for (int j = a + 1 + tid; j < b; j += NT) {
-    double t = F[j + (size_t)a * ld];
+    T t = F[j + (size_t)a * ld];
     F[j + (size_t)a * ld] = F[b + (size_t)j * ld];
     F[b + (size_t)j * ld] = t;
   }
   for (int i = b + 1 + tid; i < m; i += NT) {
-    double t = F[i + (size_t)a * ld];
+    T t = F[i + (size_t)a * ld];
     F[i + (size_t)a * ld] = F[i + (size_t)b * ld];
     F[i + (size_t)b * ld] = t;
   }
@@ -432,9 +445,10 @@ __device__ inline void sym_swap(double* F, int ld, int m, int* vars, int* pre,
 // nc advanced; W[k0:, 0:kw] and F[:, kstart:k] hold the panel for the
 // trailing update.  mode 0: threshold pivoting with delays; mode 1:
 // Bunch-Kaufman (every column is a candidate, nothing is delayed).
-template <int NT>
-__device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
-                             int* ptype, double* W, int ldw, int nb, int mode,
+template <int NT, class T>
+__device__ void panel_factor(T* F, int ld, int m, int* vars, int* pre,
+                             int* ptype, double* dv, double* dv2, double* W, int ldw,
+                             int nb, int mode,
                              double ustab,
                              double ptol, int& k, int& nc, int& kw_out,
                              FrontShared<NT>& sh) {
@@ -449,7 +463,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
     double lbest = -1.0, lrest = 0.0;
     int lidx = 0x7fffffff;
     for (int i = k + tid; i < m; i += NT) {
-      double v = F[i + (size_t)k * ld];
+      double v = (double)F[i + (size_t)k * ld];
       v = sub_dot(v, F + i + (size_t)kstart * ld, ld, sh.wr, kw);
       W[i + (size_t)kw * ldw] = v;
       if (i == k) sh.wkk = v;
@@ -502,7 +516,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
       double* W1 = W + (size_t)(kw + 1) * ldw;
       double lrow = 0.0;
       for (int i = k + tid; i < m; i += NT) {
-        double v = i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld];
+        double v = (double)(i < imax ? F[imax + (size_t)i * ld] : F[i + (size_t)imax * ld]);
         v = sub_dot(v, F + i + (size_t)kstart * ld, ld, sh.wr2, kw);
         W1[i] = v;
         if (i == imax) sh.wii = v;
@@ -548,7 +562,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
     }
     if (step == 0) {  // delay candidate k (threshold mode only)
       if (k < nc - 1) {
-        sym_swap<NT>(F, ld, m, vars, pre, k, nc - 1);
+        sym_swap<NT, T>(F, ld, m, vars, pre, k, nc - 1);
         for (int c = tid; c < kw; c += NT) {
           double t = W[k + (size_t)c * ldw];
           W[k + (size_t)c * ldw] = W[nc - 1 + (size_t)c * ldw];
@@ -561,7 +575,7 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
     }
     const int ksrc = k + step - 1;
     if (kpiv != ksrc) {
-      sym_swap<NT>(F, ld, m, vars, pre, ksrc, kpiv);
+      sym_swap<NT, T>(F, ld, m, vars, pre, ksrc, kpiv);
       for (int c = tid; c < kw + step; c += NT) {
         double t = W[ksrc + (size_t)c * ldw];
         W[ksrc + (size_t)c * ldw] = W[kpiv + (size_t)c * ldw];
@@ -574,10 +588,11 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
       const double dinv = (zero_col || d == 0.0) ? 0.0 : 1.0 / d;
       for (int i = k + tid; i < m; i += NT) {
         double v = W[i + (size_t)kw * ldw];
-        F[i + (size_t)k * ld] = (i == k || zero_col) ? v : v * dinv;
+        F[i + (size_t)k * ld] = (T)((i == k || zero_col) ? v : v * dinv);
       }
       if (tid == 0) {
         ptype[k] = 1;
+        dv[k] = d;
         if (d >= 0.0) sh.npos++;
         else sh.nneg++;
       }
@@ -590,13 +605,16 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
       const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
       for (int i = k + 2 + tid; i < m; i += NT) {
         double a = W[i + (size_t)kw * ldw], b = W[i + (size_t)(kw + 1) * ldw];
-        F[i + (size_t)k * ld] = a * i00 + b * i01;
-        F[i + (size_t)(k + 1) * ld] = a * i01 + b * i11;
+        F[i + (size_t)k * ld] = (T)(a * i00 + b * i01);
+        F[i + (size_t)(k + 1) * ld] = (T)(a * i01 + b * i11);
       }
       if (tid == 0) {
-        F[k + (size_t)k * ld] = d11;
-        F[k + 1 + (size_t)k * ld] = d21;
-        F[k + 1 + (size_t)(k + 1) * ld] = d22;
+        F[k + (size_t)k * ld] = (T)d11;
+        F[k + 1 + (size_t)k * ld] = (T)d21;
+        F[k + 1 + (size_t)(k + 1) * ld] = (T)d22;
+        dv[k] = d11;
+        dv2[k] = d21;
+        dv[k + 1] = d22;
         vars[k] = -(vars[k] + 1);
         vars[k + 1] = -(vars[k + 1] + 1);
         ptype[k] = 2;
@@ -620,8 +638,8 @@ __device__ void panel_factor(double* F, int ld, int m, int* vars, int* pre,
 
 // F[k:m, k:m] (lower) -= F[k:m, kstart:kstart+kw] * W[k:m, 0:kw]^T, one
 // workgroup, 4x4 register tiles.
-template <int NT>
-__device__ void trailing_update_wg(double* F, int ld, int m, const double* W,
+template <int NT, class T>
+__device__ void trailing_update_wg(T* F, int ld, int m, const double* W,
                                    int ldw, int kstart, int k, int kw) {
   const int nt = m - k;
   if (nt <= 0 || kw == 0) return;
@@ -644,11 +662,11 @@ __device__ void trailing_update_wg(double* F, int ld, int m, const double* W,
 #pragma unroll
       for (int cc = 0; cc < 4; cc++) {
         const bool okc = c0 + cc < kw;
-        const double* Lc = F + (size_t)(kstart + c0 + cc) * ld;
+        const T* Lc = F + (size_t)(kstart + c0 + cc) * ld;
         const double* Wc = W + (size_t)(c0 + cc) * ldw;
 #pragma unroll
         for (int a = 0; a < 4; a++) {
-          l[cc][a] = (okc && i0 + a < m) ? Lc[i0 + a] : 0.0;
+          l[cc][a] = (okc && i0 + a < m) ? (double)Lc[i0 + a] : 0.0;
           w[cc][a] = (okc && j0 + a < m) ? Wc[j0 + a] : 0.0;
         }
       }
@@ -664,7 +682,8 @@ __device__ void trailing_update_wg(double* F, int ld, int m, const double* W,
 #pragma unroll
       for (int b = 0; b < 4; b++) {
         int i = i0 + a, j = j0 + b;
-        if (i < m && j < m && i >= j) F[i + (size_t)j * ld] -= acc[a][b];
+        if (i < m && j < m && i >= j)
+          F[i + (size_t)j * ld] = (T)((double)F[i + (size_t)j * ld] - acc[a][b]);
       }
   }
   __syncthreads();
@@ -672,26 +691,28 @@ __device__ void trailing_update_wg(double* F, int ld, int m, const double* W,
 
 // Cholesky mode (include/sparse_ldl.h:2010-2044): right-looking, one
 // workgroup; used by the reference's tests only.
-template <int NT>
-__device__ void cholesky_front(double* F, int ld, int m, int fs) {
+template <int NT, class T>
+__device__ void cholesky_front(T* F, int ld, int m, int fs) {
   const int tid = threadIdx.x;
   for (int k = 0; k < fs; k++) {
-    const double d = sqrt(F[k + (size_t)k * ld]);
+    const double d = sqrt((double)F[k + (size_t)k * ld]);
     __syncthreads();
-    for (int i = k + 1 + tid; i < m; i += NT) F[i + (size_t)k * ld] /= d;
-    if (tid == 0) F[k + (size_t)k * ld] = d;
+    for (int i = k + 1 + tid; i < m; i += NT) F[i + (size_t)k * ld] = (T)((double)F[i + (size_t)k * ld] / d);
+    if (tid == 0) F[k + (size_t)k * ld] = (T)d;
     __syncthreads();
     const int nt = m - k - 1;
     for (int e = tid; e < nt * nt; e += NT) {
       int jj = e / nt, ii = e % nt;
       if (ii < jj) continue;
       int i = k + 1 + ii, j = k + 1 + jj;
-      F[i + (size_t)j * ld] -= F[i + (size_t)k * ld] * F[j + (size_t)k * ld];
+      F[i + (size_t)j * ld] = (T)((double)F[i + (size_t)j * ld] -
+                                  (double)F[i + (size_t)k * ld] * (double)F[j + (size_t)k * ld]);
     }
     __syncthreads();
   }
 }
 
+template <class T>
 __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
   constexpr int NT = kTPB;
   const int p = nodes[blockIdx.x];
@@ -701,10 +722,12 @@ __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
   const int m = P.m[p];
   const int fs = P.ns[p] + P.din[p];
   const int mode = P.mode[p];
-  double* F = P.F + P.foff[p];
+  T* F = front_ptr<T>(P, p);
   int* vars = P.vars + P.ioff[p];
   int* pre = P.pre + P.ioff[p];
   int* ptype = P.ptype + P.ioff[p];
+  double* dv = P.dv + P.ioff[p];
+  double* dv2 = P.dv2 + P.ioff[p];
   double* W = P.W + P.woff[p];
   const int ldw = P.cap[p];
   if (tid == 0) {
@@ -714,8 +737,11 @@ __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
   }
   __syncthreads();
   if (mode == 2) {
-    cholesky_front<NT>(F, m, m, fs);
-    for (int j = tid; j < fs; j += NT) ptype[j] = 1;
+    cholesky_front<NT, T>(F, m, m, fs);
+    for (int j = tid; j < fs; j += NT) {
+      ptype[j] = 1;
+      dv[j] = (double)F[j + (size_t)j * m];
+    }
     if (tid == 0) {
       P.npiv[p] = fs;
       P.ndel[p] = 0;
@@ -732,11 +758,10 @@ __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
   while (k < nc) {
     const int kstart = k;
     int kw = 0;
-    panel_factor<NT>(F, m, m, vars, pre, ptype, W, ldw, nb, mode, P.ustab, P.ptol, k, nc,
-                     kw, sh);
+    panel_factor<NT, T>(F, m, m, vars, pre, ptype, dv, dv2, W, ldw, nb, mode, P.ustab,
+                        P.ptol, k, nc, kw, sh);
     // rows of W below k hold L*D for the panel columns
-    trailing_update_wg<NT>(F, m, m, W, ldw, kstart, k, kw);
-    (void)kstart;
+    trailing_update_wg<NT, T>(F, m, m, W, ldw, kstart, k, kw);
   }
   if (tid == 0) {
     P.npiv[p] = nc;
@@ -751,11 +776,11 @@ __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
 // Small fronts: assembled straight into LDS, factored there, written back
 // once.  LDS layout: F (m x m, ld m) then W (m x kNB, ld m).
 // --------------------------------------------------------------------------
-template <int NT>
+template <int NT, class T>
 __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
   extern __shared__ double lds[];
   __shared__ FrontShared<NT> sh;
-  __shared__ ChildView s_cv[kChildCache];
+  __shared__ ChildView<T> s_cv[kChildCache];
   const int p = nodes[blockIdx.x];
   if (P.stat[p]) return;
   const int tid = threadIdx.x;
@@ -764,10 +789,12 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
   const int c0 = P.chptr[p], nch = P.chptr[p + 1] - c0;
   const bool cached = nch <= kChildCache;
   if (cached)
-    for (int t = tid; t < nch; t += NT) s_cv[t] = load_child(P, P.ch[c0 + t]);
+    for (int t = tid; t < nch; t += NT) s_cv[t] = load_child<T>(P, P.ch[c0 + t]);
   int* vars = P.vars + P.ioff[p];
   int* pre = P.pre + P.ioff[p];
   int* ptype = P.ptype + P.ioff[p];
+  double* dv = P.dv + P.ioff[p];
+  double* dv2 = P.dv2 + P.ioff[p];
   {
     const int vp = P.vptr[p], cp = P.cptr[p];
     for (int i = tid; i < ns; i += NT) vars[i] = P.node_to_var[vp + i];
@@ -786,22 +813,23 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
     sh.n2 = 0;
   }
   __syncthreads();
-  double* F = lds;
-  double* W = lds + (size_t)m * m;
+  // W (FP64) first, then F (T)
+  double* W = lds;
+  T* F = (T*)(lds + (size_t)m * kNB);
   // pull the children's contributions into the lower triangle
   for (int e = tid; e < m * m; e += NT) {
     const int j = e / m, i = e - j * m;
     double v = 0.0;
     if (i >= j) {
       for (int q = 0; q < nch; q++) {
-        ChildView cv = cached ? s_cv[q] : load_child(P, P.ch[c0 + q]);
+        ChildView<T> cv = cached ? s_cv[q] : load_child<T>(P, P.ch[c0 + q]);
         const int rj = child_row(P, cv, j, ns, din);
         if (rj < 0) continue;
         const int ri = child_row(P, cv, i, ns, din);
-        if (ri >= 0) v += cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
+        if (ri >= 0) v += (double)cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
       }
     }
-    F[e] = v;
+    F[e] = (T)v;
   }
   __syncthreads();
   const int vp = P.vptr[p];
@@ -809,16 +837,21 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
     const int g = vp + j;
     for (int e = P.am_ptr[g] + tid; e < P.am_ptr[g + 1]; e += NT) {
       const int u = P.am_row[e];
-      F[(u < ns ? u : u + din) + (size_t)j * m] += P.vals[P.am_src[e]];
+      const size_t r = (u < ns ? u : u + din) + (size_t)j * m;
+      F[r] = (T)((double)F[r] + P.vals[P.am_src[e]]);
     }
   }
   __syncthreads();
   if (P.diag)
-    for (int j = tid; j < ns; j += NT) F[(size_t)j * m + j] += P.diag[P.node_to_var[vp + j]];
+    for (int j = tid; j < ns; j += NT)
+      F[(size_t)j * m + j] = (T)((double)F[(size_t)j * m + j] + P.diag[P.node_to_var[vp + j]]);
   __syncthreads();
   if (mode == 2) {
-    cholesky_front<NT>(F, m, m, fs);
-    for (int j = tid; j < fs; j += NT) ptype[j] = 1;
+    cholesky_front<NT, T>(F, m, m, fs);
+    for (int j = tid; j < fs; j += NT) {
+      ptype[j] = 1;
+      dv[j] = (double)F[j + (size_t)j * m];
+    }
     if (tid == 0) {
       P.npiv[p] = fs;
       P.ndel[p] = 0;
@@ -830,9 +863,9 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
     while (k < nc) {
       const int kstart = k;
       int kw = 0;
-      panel_factor<NT>(F, m, m, vars, pre, ptype, W, m, nb, mode, P.ustab, P.ptol, k, nc,
-                       kw, sh);
-      trailing_update_wg<NT>(F, m, m, W, m, kstart, k, kw);
+      panel_factor<NT, T>(F, m, m, vars, pre, ptype, dv, dv2, W, m, nb, mode, P.ustab,
+                          P.ptol, k, nc, kw, sh);
+      trailing_update_wg<NT, T>(F, m, m, W, m, kstart, k, kw);
     }
     if (tid == 0) {
       P.npiv[p] = nc;
@@ -843,7 +876,7 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
     }
   }
   __syncthreads();
-  double* Fg = P.F + P.foff[p];
+  T* Fg = front_ptr<T>(P, p);
   for (int e = tid; e < m * m; e += NT) Fg[e] = F[e];
 }
 
@@ -851,7 +884,7 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
 // Large fronts: one panel per launch on one workgroup (state in P.ps), the
 // trailing update spread over many workgroups (k_update).
 // --------------------------------------------------------------------------
-template <int NT>
+template <int NT, class T>
 __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
   const int p = nodes[blockIdx.x];
   if (P.stat[p]) return;
@@ -871,9 +904,9 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
     const int m = P.m[p];
     const int mode = P.mode[p];
     const int nb = max(2, min(kNBLarge, m));
-    panel_factor<NT>(P.F + P.foff[p], m, m, P.vars + P.ioff[p], P.pre + P.ioff[p],
-                 P.ptype + P.ioff[p], P.W + P.woff[p], P.cap[p], nb, mode, P.ustab,
-                 P.ptol, k, nc, kw, sh);
+    panel_factor<NT, T>(front_ptr<T>(P, p), m, m, P.vars + P.ioff[p], P.pre + P.ioff[p],
+                        P.ptype + P.ioff[p], P.dv + P.ioff[p], P.dv2 + P.ioff[p],
+                        P.W + P.woff[p], P.cap[p], nb, mode, P.ustab, P.ptol, k, nc, kw, sh);
     if (tid == 0 && k >= nc) {
       const int fs = P.ns[p] + P.din[p];
       P.npiv[p] = nc;
@@ -896,7 +929,6 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
   }
 }
 
-typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // F[k:m, k:m] (lower) -= F[k:m, kstart:kstart+kw] * W[k:m, 0:kw]^T with
 // v_mfma_f64_16x16x4f64.  Workgroup = 64 x 64 tile of the region starting
@@ -904,8 +936,12 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // MFMA operand layout (checked on gfx950): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16] and receives D[l/16 + 4r][l%16], r = 0..3 (the FP64 16x16x4
 // output interleaves rows by 4, unlike the FP32 16x16x4 one).
+// FP32 fronts use v_mfma_f32_16x16x4f32 (FP32 accumulate), whose output is
+// D[4(l/16) + r][l%16] (also checked on gfx950).
+template <class T>
 __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
                                                  const int* wtile, int base) {
+  typedef T vec4 __attribute__((ext_vector_type(4)));
   const int p = wnode[blockIdx.x];
   if (P.stat[p]) return;
   const PanelState st = P.ps[p];
@@ -922,25 +958,28 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
   const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
   const int ri = i0 + 32 * (w >> 1), cj = j0 + 32 * (w & 1);
   if (ri + 32 <= cj) return;  // strictly above the diagonal
-  double* F = P.F + P.foff[p];
+  T* F = front_ptr<T>(P, p);
   const double* W = P.W + P.woff[p];
   const int ldw = P.cap[p];
   const int lr = l & 15, lk = l >> 4;
+  constexpr bool f64 = sizeof(T) == 8;
+  // output row of register r in this lane
+  auto orow = [&](int r) { return f64 ? lk + 4 * r : 4 * lk + r; };
   // Accumulator starts from the F tile; the A operand is -L, so the MFMA
   // chain produces F - L W^T directly.  All loads are issued up front.
-  dbl4 acc[2][2];
+  vec4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = ri + 16 * a + lk + 4 * r;
+        const int i = ri + 16 * a + orow(r);
         const int j = cj + 16 * b + lr;
-        acc[a][b][r] = (i < m && j < m) ? F[i + (size_t)j * m] : 0.0;
+        acc[a][b][r] = (i < m && j < m) ? F[i + (size_t)j * m] : (T)0;
       }
   constexpr int KS = kNBLarge / 4;  // k-steps of 4 (kw <= kNBLarge)
-  double av[KS][2], bv[KS][2];
+  T av[KS][2], bv[KS][2];
 #pragma unroll
   for (int q = 0; q < KS; q++) {
     const int c = 4 * q + lk;
@@ -948,9 +987,9 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
 #pragma unroll
     for (int a = 0; a < 2; a++) {
       const int row = ri + 16 * a + lr;
-      av[q][a] = (ok && row < m) ? -F[row + (size_t)(kstart + c) * m] : 0.0;
+      av[q][a] = (ok && row < m) ? -F[row + (size_t)(kstart + c) * m] : (T)0;
       const int col = cj + 16 * a + lr;
-      bv[q][a] = (ok && col < m) ? W[col + (size_t)c * ldw] : 0.0;
+      bv[q][a] = (ok && col < m) ? (T)W[col + (size_t)c * ldw] : (T)0;
     }
   }
 #pragma unroll
@@ -959,8 +998,12 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q][a], bv[q][b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < 2; b++) {
+          if constexpr (f64)
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q][a], bv[q][b], acc[a][b], 0, 0, 0);
+          else
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][a], bv[q][b], acc[a][b], 0, 0, 0);
+        }
     }
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -968,7 +1011,7 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = ri + 16 * a + lk + 4 * r;
+        const int i = ri + 16 * a + orow(r);
         const int j = cj + 16 * b + lr;
         if (i < m && j < m && i >= j && j >= k) F[i + (size_t)j * m] = acc[a][b][r];
       }
@@ -990,6 +1033,7 @@ __device__ inline int block_end(const int* ptype, int jb, int npiv, int chol) {
   return je;
 }
 
+template <class T>
 __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
                                                   double* x, int ldx, int nrhs,
                                                   long long ustride, int chol) {
@@ -999,10 +1043,12 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
   const int m = P.m[p], npiv = P.npiv[p];
   const int ns = P.ns[p], din = P.din[p];
   const int fs = ns + din;
-  const double* L = P.F + P.foff[p];
+  const T* L = front_ptr<T>(P, p);
   const int* vars = P.vars + P.ioff[p];
   const int* pre = P.pre + P.ioff[p];
   const int* ptype = P.ptype + P.ioff[p];
+  const double* dv = P.dv + P.ioff[p];
+  const double* dv2 = P.dv2 + P.ioff[p];
   const int c0 = P.chptr[p], c1 = P.chptr[p + 1];
   const int vp = P.vptr[p];
   for (int r = 0; r < nrhs; r++) {
@@ -1012,7 +1058,7 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
       double v = pi < ns ? xr[P.node_to_var[vp + pi]] : 0.0;
       for (int q = c0; q < c1; q++) {
         const int c = P.ch[q];
-        ChildView cv = load_child(P, c);
+        ChildView<T> cv = load_child<T>(P, c);
         int rc = child_row(P, cv, pi, ns, din);
         if (rc >= 0) v += P.U[P.uoff[c] + (long long)r * ustride + (rc - cv.npiv)];
       }
@@ -1025,13 +1071,13 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
       double* Lb = t + P.lds_t;  // kSB x kSB block, column major
       for (int e = tid; e < bs * bs; e += kTPB) {
         const int c = e / bs, rr = e - c * bs;
-        Lb[rr + c * kSB] = L[jb + rr + (size_t)(jb + c) * m];
+        Lb[rr + c * kSB] = (double)L[jb + rr + (size_t)(jb + c) * m];
       }
       __syncthreads();
       if (w == 0) {
         const int row = jb + lane;
         double tr = lane < bs ? t[row] : 0.0;
-        const double dr = (chol && lane < bs) ? Lb[lane + lane * kSB] : 1.0;
+        const double dr = (chol && lane < bs) ? dv[row] : 1.0;
         for (int c = 0; c < bs; c++) {
           const int ty = chol ? 1 : ptype[jb + c];
           if (ty == 1) {
@@ -1061,10 +1107,9 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
       for (int j = tid; j < npiv; j += kTPB) {
         const int ty = ptype[j];
         if (ty == 1) {
-          t[j] = t[j] / L[j + (size_t)j * m];
+          t[j] = t[j] / dv[j];
         } else if (ty == 2) {
-          const double d11 = L[j + (size_t)j * m], d21 = L[j + 1 + (size_t)j * m],
-                       d22 = L[j + 1 + (size_t)(j + 1) * m];
+          const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
           const double inv = 1.0 / (d11 * d22 - d21 * d21);
           const double a = t[j], b = t[j + 1];
           t[j] = inv * (d22 * a - d21 * b);
@@ -1081,6 +1126,7 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
 // Backward: t1 = x[pivots] - L21^T x[rows below], then the L11^T back solve
 // in blocks from the bottom (dot products over the rows below a block by
 // the whole workgroup, the diagonal block by one wave in registers).
+template <class T>
 __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
                                                    double* x, int ldx, int nrhs,
                                                    int chol) {
@@ -1089,9 +1135,10 @@ __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
   const int tid = threadIdx.x;
   const int m = P.m[p], npiv = P.npiv[p];
   const int fs = P.ns[p] + P.din[p];
-  const double* L = P.F + P.foff[p];
+  const T* L = front_ptr<T>(P, p);
   const int* vars = P.vars + P.ioff[p];
   const int* ptype = P.ptype + P.ioff[p];
+  const double* dv = P.dv + P.ioff[p];
   const int cp = P.cptr[p];
   const int w = tid / kWave, lane = tid % kWave;
   for (int r = 0; r < nrhs; r++) {
@@ -1126,13 +1173,13 @@ __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
       double* Lb = t + P.lds_t;
       for (int e = tid; e < bs * bs; e += kTPB) {
         const int c = e / bs, rr = e - c * bs;
-        Lb[rr + c * kSB] = L[jb + rr + (size_t)(jb + c) * m];
+        Lb[rr + c * kSB] = (double)L[jb + rr + (size_t)(jb + c) * m];
       }
       __syncthreads();
       if (w == 0) {
         // lane c owns column jb + c of the block
         const int col = jb + lane;
-        const double dc = (chol && lane < bs) ? Lb[lane + lane * kSB] : 1.0;
+        const double dc = (chol && lane < bs) ? dv[col] : 1.0;
         double tv = lane < bs ? t[col] : 0.0;
         double acc = 0.0;
         for (int i = bs - 1; i >= 0; i--) {
@@ -1267,7 +1314,8 @@ struct Handle {
   std::vector<int> upd_ptr, upd_node, upd_tile;  // upd_ptr per (level, panel)
   std::vector<int> upd_lvl;                      // first upd_ptr slot of a level
   DBuf<int> d_small_nodes, d_large_nodes, d_upd_node, d_upd_tile;
-  DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_norm;
+  DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_norm, d_dv, d_dv2;
+  int esize = 8;  // bytes per front entry: 8 (FP64) or 4 (FP32 factor)
   DBuf<int> d_vars, d_pre, d_ptype;
   bool has_diag = false;
   bool factored = false;
@@ -1507,7 +1555,9 @@ static void upload_static(Handle& H) {
   H.d_large_nodes.upload(H.large_nodes);
   H.d_upd_node.upload(H.upd_node);
   H.d_upd_tile.upload(H.upd_tile);
-  H.d_F.alloc((size_t)H.f_total);
+  H.d_F.alloc(((size_t)H.f_total * H.esize + 7) / 8);
+  H.d_dv.alloc((size_t)H.i_total);
+  H.d_dv2.alloc((size_t)H.i_total);
   H.d_vars.alloc((size_t)H.i_total);
   H.d_pre.alloc((size_t)H.i_total);
   H.d_ptype.alloc((size_t)H.i_total);
@@ -1545,7 +1595,9 @@ static DevPlan dev_plan(Handle& H) {
   P.need = H.d_need.p;
   P.cnt = H.d_cnt.p;
   P.ps = H.d_ps.p;
-  P.F = H.d_F.p;
+  P.F = (void*)H.d_F.p;
+  P.dv = H.d_dv.p;
+  P.dv2 = H.d_dv2.p;
   P.vars = H.d_vars.p;
   P.pre = H.d_pre.p;
   P.ptype = H.d_ptype.p;
@@ -1623,7 +1675,8 @@ static void prof_collect(Handle& H) {
   H.prof_cls.clear();
 }
 
-static void run_factor_levels(Handle& H, hipStream_t st) {
+template <class T>
+static void run_factor_levels_t(Handle& H, hipStream_t st) {
   DevPlan P = dev_plan(H);
   const int nlev = (int)H.lvl_ptr.size() - 1;
   for (int l = 0; l < nlev; l++) {
@@ -1636,7 +1689,7 @@ static void run_factor_levels(Handle& H, hipStream_t st) {
     int a0 = H.asm_lvl_ptr[l], a1 = H.asm_lvl_ptr[l + 1];
     if (a1 > a0) {
       ProfScope ps_(H, st, KC_ASSEMBLE);
-      hipLaunchKernelGGL(k_assemble, dim3(a1 - a0), dim3(kTPB), 0, st, P,
+      hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
                          H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
     }
     const int d0 = H.lds_ptr[l], d1 = H.lds_ptr[l + 1];
@@ -1644,32 +1697,37 @@ static void run_factor_levels(Handle& H, hipStream_t st) {
       const int c = H.lds_cap[l];
       const size_t bytes = sizeof(double) * ((size_t)c * c + (size_t)c * kNB);
       ProfScope ps_(H, st, KC_FRONT_LDS);
-      hipLaunchKernelGGL(k_front_lds<kTPB>, dim3(d1 - d0), dim3(kTPB), bytes, st, P,
+      hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(d1 - d0), dim3(kTPB), bytes, st, P,
                          H.d_lds_nodes.p + d0);
     }
     const int s0 = H.small_ptr[l], s1 = H.small_ptr[l + 1];
     if (s1 > s0) {
       ProfScope ps_(H, st, KC_FACTOR);
-      hipLaunchKernelGGL(k_factor, dim3(s1 - s0), dim3(kTPB), 0, st, P,
+      hipLaunchKernelGGL(k_factor<T>, dim3(s1 - s0), dim3(kTPB), 0, st, P,
                          H.d_small_nodes.p + s0);
     }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     for (int t = 0; t < H.large_panels[l]; t++) {
       {
         ProfScope ps_(H, st, KC_PANEL);
-        hipLaunchKernelGGL(k_panel<kPanelThreads>, dim3(g1 - g0), dim3(kPanelThreads), 0, st,
+        hipLaunchKernelGGL((k_panel<kPanelThreads, T>), dim3(g1 - g0), dim3(kPanelThreads), 0, st,
                            P, H.d_large_nodes.p + g0);
       }
       const int slot = H.upd_lvl[l] + t;
       const int u0 = H.upd_ptr[slot], u1 = H.upd_ptr[slot + 1];
       if (u1 > u0) {
         ProfScope ps_(H, st, KC_UPDATE);
-        hipLaunchKernelGGL(k_update, dim3(u1 - u0), dim3(kTPB), 0, st, P,
+        hipLaunchKernelGGL(k_update<T>, dim3(u1 - u0), dim3(kTPB), 0, st, P,
                            H.d_upd_node.p + u0, H.d_upd_tile.p + u0, t * (kNBLarge - 2));
       }
     }
   }
   HIPCHECK(hipGetLastError());
+}
+
+static void run_factor_levels(Handle& H, hipStream_t st) {
+  if (H.esize == 4) run_factor_levels_t<float>(H, st);
+  else run_factor_levels_t<double>(H, st);
 }
 
 // Device memory is claimed on the first factorization, so that the
@@ -1768,7 +1826,8 @@ static void ensure_u(Handle& H, int nrhs) {
   H.u_nrhs = nrhs;
 }
 
-static void run_solve_levels(Handle& H, double* dx, int ldx, int nrhs, hipStream_t st) {
+template <class T>
+static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStream_t st) {
   ensure_u(H, nrhs);
   DevPlan P = dev_plan(H);
   const int nlev = (int)H.lvl_ptr.size() - 1;
@@ -1780,16 +1839,21 @@ static void run_solve_levels(Handle& H, double* dx, int ldx, int nrhs, hipStream
   for (int l = 0; l < nlev; l++) {
     int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
     ProfScope ps_(H, st, KC_FORWARD);
-    hipLaunchKernelGGL(k_forward, dim3(n1 - n0), dim3(kTPB), lds, st, P,
+    hipLaunchKernelGGL(k_forward<T>, dim3(n1 - n0), dim3(kTPB), lds, st, P,
                        H.d_lvl_nodes.p + n0, dx, ldx, nrhs, (long long)H.u_total, chol);
   }
   for (int l = nlev - 1; l >= 0; l--) {
     int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
     ProfScope ps_(H, st, KC_BACKWARD);
-    hipLaunchKernelGGL(k_backward, dim3(n1 - n0), dim3(kTPB), lds, st, P,
+    hipLaunchKernelGGL(k_backward<T>, dim3(n1 - n0), dim3(kTPB), lds, st, P,
                        H.d_lvl_nodes.p + n0, dx, ldx, nrhs, chol);
   }
   HIPCHECK(hipGetLastError());
+}
+
+static void run_solve_levels(Handle& H, double* dx, int ldx, int nrhs, hipStream_t st) {
+  if (H.esize == 4) run_solve_levels_t<float>(H, dx, ldx, nrhs, st);
+  else run_solve_levels_t<double>(H, dx, ldx, nrhs, st);
 }
 
 static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
@@ -1860,8 +1924,9 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   if (o) H.opt = *o;
   else kkt_ldl_default_options(&H.opt);
   if (n <= 0) throw Error(-1, "n must be positive");
-  if (H.opt.precision != KKT_LDL_FP64)
-    throw Error(-8, "only the FP64 factor is available in this build");
+  if (H.opt.precision != KKT_LDL_FP64 && H.opt.precision != KKT_LDL_FP32_REFINE)
+    throw Error(-1, "unknown precision");
+  H.esize = (H.opt.precision == KKT_LDL_FP32_REFINE && H.opt.solver_type == KKT_LDL_LDL) ? 4 : 8;
   H.n = n;
   H.nnz = rowp[n];
   H.rowp.assign(rowp, rowp + n + 1);
@@ -2067,7 +2132,7 @@ int kkt_ldl_get_node_info(kkt_ldl_t h, int* npiv, int* ndel, int* n2x2, int* roo
 int kkt_ldl_get_stats(kkt_ldl_t h, kkt_ldl_stats* st) {
   return guarded([&] {
     if (!h || !st) throw Error(-1, "null argument");
-    h->h.stats.front_bytes = h->h.f_total * (long long)sizeof(double);
+    h->h.stats.front_bytes = h->h.f_total * (long long)h->h.esize;
     *st = h->h.stats;
     return 0;
   });

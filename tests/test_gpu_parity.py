@@ -157,3 +157,45 @@ def test_full_size_workloads_vs_oracle(name, kw):
     x = P.rhs.copy()
     s.solve(x, refine=True)
     assert _relres(P.matrix(), x, P.rhs) <= 1e-10
+
+
+# --- FP32 factor + FP64 refinement ----------------------------------------
+
+LDLC = [c for c in CASES if int(c["solver_type"]) == SolverType.LDL]
+
+
+@pytest.mark.parametrize("c", LDLC, ids=[str(c["name"]) for c in LDLC])
+def test_fp32_factor_refined_vs_reference(c):
+    """FP32 fronts, FP64 pivot decisions and D: identical inertia; FP64
+    refinement reaches <= 1e-10 where the reference's FP64 solve does."""
+    from amigo_amd.ldl import FP32_REFINE
+    s = _engine(c, precision=FP32_REFINE, refine_tol=1e-10, max_refine=30)
+    diag = c["diag"] if bool(c["has_diag"]) else None
+    assert s.factor(diag) == 0
+    assert s.get_inertia() == tuple(int(v) for v in c["inertia"])
+    K = _K(c)
+    r_ref = _relres(K, c["x"], c["rhs"])
+    x = np.array(c["rhs"], dtype=np.float64, order="F", copy=True)
+    s.solve(x, refine=True)
+    if r_ref <= 1e-10:
+        assert _relres(K, x, c["rhs"]) <= 1e-10
+        assert np.abs(x - c["x"]).max() <= 1e-6 * np.abs(c["x"]).max()
+
+
+@pytest.mark.parametrize("name,kw", [("poisson2d", {}), ("cartpole", {})])
+def test_fp32_full_size_inertia_and_residual(name, kw):
+    from amigo_amd.ldl import FP32_REFINE
+    P = W.GENERATORS[name](**kw)
+    s64 = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
+                    order=OrderingType.NESTED_DISSECTION)
+    s64.factor(P.diag)
+    s32 = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
+                    order=OrderingType.NESTED_DISSECTION, precision=FP32_REFINE,
+                    max_refine=30)
+    s32.factor(P.diag)
+    assert s32.get_inertia() == s64.get_inertia()
+    x = P.rhs.copy()
+    s32.solve(x, refine=True)
+    st = s32.stats()
+    print("fp32 refine steps", st["refine_steps"], "residual", st["residual"])
+    assert _relres(P.matrix(), x, P.rhs) <= 1e-10
