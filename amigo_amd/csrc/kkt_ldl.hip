@@ -101,6 +101,7 @@ struct DevPlan {
   const int* am_ptr;   // per own column (global index g in node_to_var order)
   const int* am_src;   // CSR index into vals
   const int* am_row;   // nominal row in the front
+  const int* am_col;   // own column of the entry inside its front
   int lds_t;           // solve kernels: doubles of LDS before the staged block
   // dynamic per node
   int* din;            // delayed pivots received
@@ -337,13 +338,12 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
   __syncthreads();
   // original entries of the own columns in [j0, j1): distinct positions
   const int vp = P.vptr[p];
-  for (int j = j0; j < min(j1, ns); j++) {
-    const int g = vp + j;
-    T* Fj = F + (size_t)j * m;
-    for (int e = P.am_ptr[g] + tid; e < P.am_ptr[g + 1]; e += kTPB) {
-      const int u = P.am_row[e];
-      const int r = u < ns ? u : u + din;
-      Fj[r] = (T)((double)Fj[r] + P.vals[P.am_src[e]]);
+  if (j0 < ns) {
+    const int e0 = P.am_ptr[vp + j0], e1 = P.am_ptr[vp + min(j1, ns)];
+    for (int e = e0 + tid; e < e1; e += kTPB) {
+      const int u = P.am_row[e], j = P.am_col[e];
+      const size_t r = (u < ns ? u : u + din) + (size_t)j * m;
+      F[r] = (T)((double)F[r] + P.vals[P.am_src[e]]);
     }
   }
   __syncthreads();
@@ -790,11 +790,14 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
   const bool cached = nch <= kChildCache;
   if (cached)
     for (int t = tid; t < nch; t += NT) s_cv[t] = load_child<T>(P, P.ch[c0 + t]);
-  int* vars = P.vars + P.ioff[p];
-  int* pre = P.pre + P.ioff[p];
-  int* ptype = P.ptype + P.ioff[p];
-  double* dv = P.dv + P.ioff[p];
-  double* dv2 = P.dv2 + P.ioff[p];
+  // LDS: W (m x kNB, FP64) | dv | dv2 | F (m x m, T) | vars | pre | ptype
+  double* W = lds;
+  double* dv = W + (size_t)m * kNB;
+  double* dv2 = dv + m;
+  T* F = (T*)(dv2 + m);
+  int* vars = (int*)(F + (size_t)m * m);
+  int* pre = vars + m;
+  int* ptype = pre + m;
   {
     const int vp = P.vptr[p], cp = P.cptr[p];
     for (int i = tid; i < ns; i += NT) vars[i] = P.node_to_var[vp + i];
@@ -807,15 +810,13 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
       for (int t = tid; t < nd; t += NT) vars[ns + off + t] = cv[t];
     }
   }
+  for (int i = tid; i < m; i += NT) ptype[i] = 0;
   if (tid == 0) {
     sh.npos = 0;
     sh.nneg = 0;
     sh.n2 = 0;
   }
   __syncthreads();
-  // W (FP64) first, then F (T)
-  double* W = lds;
-  T* F = (T*)(lds + (size_t)m * kNB);
   // pull the children's contributions into the lower triangle
   for (int e = tid; e < m * m; e += NT) {
     const int j = e / m, i = e - j * m;
@@ -833,13 +834,11 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
   }
   __syncthreads();
   const int vp = P.vptr[p];
-  for (int j = 0; j < ns; j++) {
-    const int g = vp + j;
-    for (int e = P.am_ptr[g] + tid; e < P.am_ptr[g + 1]; e += NT) {
-      const int u = P.am_row[e];
-      const size_t r = (u < ns ? u : u + din) + (size_t)j * m;
-      F[r] = (T)((double)F[r] + P.vals[P.am_src[e]]);
-    }
+  // every original entry of the own columns lands on a distinct position
+  for (int e = P.am_ptr[vp] + tid; e < P.am_ptr[vp + ns]; e += NT) {
+    const int u = P.am_row[e], j = P.am_col[e];
+    const size_t r = (u < ns ? u : u + din) + (size_t)j * m;
+    F[r] = (T)((double)F[r] + P.vals[P.am_src[e]]);
   }
   __syncthreads();
   if (P.diag)
@@ -878,6 +877,14 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
   __syncthreads();
   T* Fg = front_ptr<T>(P, p);
   for (int e = tid; e < m * m; e += NT) Fg[e] = F[e];
+  const long long io = P.ioff[p];
+  for (int i = tid; i < m; i += NT) {
+    P.vars[io + i] = vars[i];
+    P.pre[io + i] = pre[i];
+    P.ptype[io + i] = ptype[i];
+    P.dv[io + i] = dv[i];
+    P.dv2[io + i] = dv2[i];
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -886,6 +893,7 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
 // --------------------------------------------------------------------------
 template <int NT, class T>
 __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
+  extern __shared__ int s_iv[];  // vars and pre of the front, staged in LDS
   const int p = nodes[blockIdx.x];
   if (P.stat[p]) return;
   __shared__ FrontShared<NT> sh;
@@ -904,9 +912,21 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
     const int m = P.m[p];
     const int mode = P.mode[p];
     const int nb = max(2, min(kNBLarge, m));
-    panel_factor<NT, T>(front_ptr<T>(P, p), m, m, P.vars + P.ioff[p], P.pre + P.ioff[p],
-                        P.ptype + P.ioff[p], P.dv + P.ioff[p], P.dv2 + P.ioff[p],
-                        P.W + P.woff[p], P.cap[p], nb, mode, P.ustab, P.ptol, k, nc, kw, sh);
+    const long long io = P.ioff[p];
+    int* vars = s_iv;
+    int* pre = s_iv + m;
+    for (int i = k + tid; i < m; i += NT) {
+      vars[i] = P.vars[io + i];
+      pre[i] = P.pre[io + i];
+    }
+    __syncthreads();
+    panel_factor<NT, T>(front_ptr<T>(P, p), m, m, vars, pre, P.ptype + io, P.dv + io,
+                        P.dv2 + io, P.W + P.woff[p], P.cap[p], nb, mode, P.ustab, P.ptol,
+                        k, nc, kw, sh);
+    for (int i = kstart + tid; i < m; i += NT) {
+      P.vars[io + i] = vars[i];
+      P.pre[io + i] = pre[i];
+    }
     if (tid == 0 && k >= nc) {
       const int fs = P.ns[p] + P.din[p];
       P.npiv[p] = nc;
@@ -929,13 +949,6 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
   }
 }
 
-
-// F[k:m, k:m] (lower) -= F[k:m, kstart:kstart+kw] * W[k:m, 0:kw]^T with
-// v_mfma_f64_16x16x4f64.  Workgroup = 64 x 64 tile of the region starting
-// at `base` (a lower bound of k); each wave a 32 x 32 quarter of it.
-// MFMA operand layout (checked on gfx950): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16] and receives D[l/16 + 4r][l%16], r = 0..3 (the FP64 16x16x4
-// output interleaves rows by 4, unlike the FP32 16x16x4 one).
 // FP32 fronts use v_mfma_f32_16x16x4f32 (FP32 accumulate), whose output is
 // D[4(l/16) + r][l%16] (also checked on gfx950).
 template <class T>
@@ -1051,16 +1064,33 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
   const double* dv2 = P.dv2 + P.ioff[p];
   const int c0 = P.chptr[p], c1 = P.chptr[p + 1];
   const int vp = P.vptr[p];
+  __shared__ ChildView<T> s_cv[kChildCache];
+  __shared__ long long s_uo[kChildCache];
+  const int nch = c1 - c0;
+  const bool cached = nch <= kChildCache;
+  if (cached)
+    for (int q = tid; q < nch; q += kTPB) {
+      s_cv[q] = load_child<T>(P, P.ch[c0 + q]);
+      s_uo[q] = P.uoff[P.ch[c0 + q]];
+    }
+  __syncthreads();
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
     for (int i = tid; i < m; i += kTPB) {
       const int pi = i < fs ? pre[i] : i;
       double v = pi < ns ? xr[P.node_to_var[vp + pi]] : 0.0;
-      for (int q = c0; q < c1; q++) {
-        const int c = P.ch[q];
-        ChildView<T> cv = load_child<T>(P, c);
+      for (int q = 0; q < nch; q++) {
+        ChildView<T> cv;
+        long long uo;
+        if (cached) {
+          cv = s_cv[q];
+          uo = s_uo[q];
+        } else {
+          cv = load_child<T>(P, P.ch[c0 + q]);
+          uo = P.uoff[P.ch[c0 + q]];
+        }
         int rc = child_row(P, cv, pi, ns, din);
-        if (rc >= 0) v += P.U[P.uoff[c] + (long long)r * ustride + (rc - cv.npiv)];
+        if (rc >= 0) v += P.U[uo + (long long)r * ustride + (rc - cv.npiv)];
       }
       t[i] = v;
     }
@@ -1078,8 +1108,9 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
         const int row = jb + lane;
         double tr = lane < bs ? t[row] : 0.0;
         const double dr = (chol && lane < bs) ? dv[row] : 1.0;
+        const int pt = (!chol && lane < bs) ? ptype[row] : 1;
         for (int c = 0; c < bs; c++) {
-          const int ty = chol ? 1 : ptype[jb + c];
+          const int ty = __shfl(pt, c);
           if (ty == 1) {
             if (chol && lane == c) tr = tr / dr;
             const double tc = __shfl(tr, c);
@@ -1182,8 +1213,9 @@ __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
         const double dc = (chol && lane < bs) ? dv[col] : 1.0;
         double tv = lane < bs ? t[col] : 0.0;
         double acc = 0.0;
+        const int pt = (!chol && lane < bs) ? ptype[col] : 1;
         for (int i = bs - 1; i >= 0; i--) {
-          const int ty = chol ? 1 : ptype[jb + i];
+          const int ty = __shfl(pt, i);
           if (ty == 1) {
             if (lane == i) tv = chol ? (tv - acc) / dc : tv - acc;
             const double ti = __shfl(tv, i);
@@ -1224,21 +1256,33 @@ __global__ void k_residual(int n, const int* rowp, const int* cols,
   }
 }
 
-// sum of squares of each column into out[c] (atomic-free: one block per column)
-__global__ void k_colnorm2(int n, const double* x, int ld, double* out) {
+// Sum of squares of each column: partial sums per (column, block), then a
+// fixed-order final sum per column (deterministic, no atomics).
+constexpr int kNormBlocks = 128;
+
+__global__ void k_colnorm2_part(int n, const double* x, int ld, double* part) {
   __shared__ double red[kTPB / kWave];
-  const int c = blockIdx.x;
+  const int c = blockIdx.y;
   const double* xc = x + (size_t)c * ld;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += kTPB) s += xc[i] * xc[i];
+  for (int i = blockIdx.x * kTPB + threadIdx.x; i < n; i += kNormBlocks * kTPB)
+    s += xc[i] * xc[i];
   for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
   if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int i = 0; i < kTPB / kWave; i++) t += red[i];
-    out[c] = t;
+    part[(size_t)c * kNormBlocks + blockIdx.x] = t;
   }
+}
+
+__global__ void k_colnorm2_final(int nrhs, const double* part, double* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  double t = 0.0;
+  for (int b = 0; b < kNormBlocks; b++) t += part[(size_t)c * kNormBlocks + b];
+  out[c] = t;
 }
 
 __global__ void k_axpy_cols(int n, double* x, const double* dx, int ld, int nrhs) {
@@ -1294,13 +1338,13 @@ struct Handle {
   // plan (host)
   std::vector<int> ns, cs, cap, mode, level, lvl_ptr, lvl_nodes;
   std::vector<long long> foff, ioff, woff, uoff, invoff;
-  std::vector<int> invnom, am_ptr, am_src, am_row;
+  std::vector<int> invnom, am_ptr, am_src, am_row, am_col;
   std::vector<int> asm_node, asm_chunk, asm_lvl_ptr;
   long long f_total = 0, i_total = 0, w_total = 0, u_total = 0;
   int max_cap = 0;
   // device
   DBuf<int> d_ns, d_cs, d_cap, d_vptr, d_cptr, d_chptr, d_ch, d_mode, d_n2v,
-      d_crows, d_invnom, d_amptr, d_amsrc, d_amrow, d_lvl_nodes, d_asm_node,
+      d_crows, d_invnom, d_amptr, d_amsrc, d_amrow, d_amcol, d_lvl_nodes, d_asm_node,
       d_asm_chunk, d_rowp, d_cols;
   DBuf<long long> d_invoff, d_foff, d_ioff, d_woff, d_uoff;
   DBuf<int> d_din, d_m, d_npiv, d_ndel, d_doff, d_stat, d_need, d_cnt;
@@ -1434,6 +1478,8 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
         H.small_nodes.push_back(k);
       }
     }
+    std::sort(H.lds_nodes.begin() + H.lds_ptr[l], H.lds_nodes.end(),
+              [&](int a, int b) { return H.cap[a] < H.cap[b]; });
     H.lds_ptr[l + 1] = (int)H.lds_nodes.size();
     H.small_ptr[l + 1] = (int)H.small_nodes.size();
     H.large_ptr[l + 1] = (int)H.large_nodes.size();
@@ -1488,6 +1534,7 @@ static void build_maps(Handle& H) {
   H.am_ptr.assign(n + 1, 0);
   H.am_src.clear();
   H.am_row.clear();
+  H.am_col.clear();
   std::vector<int> nominal(n, -1);
   for (int k = 0; k < N; k++) {
     const int ns = S.node_ptr[k + 1] - S.node_ptr[k];
@@ -1504,6 +1551,7 @@ static void build_maps(Handle& H) {
         if (pi >= pv && nominal[i] >= 0) {
           H.am_src.push_back(e);
           H.am_row.push_back(nominal[i]);
+          H.am_col.push_back(q - S.node_ptr[k]);
         }
       }
       H.am_ptr[q + 1] = (int)H.am_src.size();
@@ -1531,6 +1579,7 @@ static void upload_static(Handle& H) {
   H.d_amptr.upload(H.am_ptr);
   H.d_amsrc.upload(H.am_src);
   H.d_amrow.upload(H.am_row);
+  H.d_amcol.upload(H.am_col);
   H.d_foff.upload(H.foff);
   H.d_ioff.upload(H.ioff);
   H.d_woff.upload(H.woff);
@@ -1586,6 +1635,7 @@ static DevPlan dev_plan(Handle& H) {
   P.am_ptr = H.d_amptr.p;
   P.am_src = H.d_amsrc.p;
   P.am_row = H.d_amrow.p;
+  P.am_col = H.d_amcol.p;
   P.din = H.d_din.p;
   P.m = H.d_m.p;
   P.npiv = H.d_npiv.p;
@@ -1675,6 +1725,12 @@ static void prof_collect(Handle& H) {
   H.prof_cls.clear();
 }
 
+// LDS bytes of k_front_lds for a front of dimension c.
+static size_t lds_front_bytes(int c, size_t esize) {
+  return sizeof(double) * ((size_t)c * kNB + 2 * (size_t)c) + esize * (size_t)c * c +
+         3 * sizeof(int) * (size_t)c + 16;
+}
+
 template <class T>
 static void run_factor_levels_t(Handle& H, hipStream_t st) {
   DevPlan P = dev_plan(H);
@@ -1692,13 +1748,19 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
                          H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
     }
-    const int d0 = H.lds_ptr[l], d1 = H.lds_ptr[l + 1];
-    if (d1 > d0) {
-      const int c = H.lds_cap[l];
-      const size_t bytes = sizeof(double) * ((size_t)c * c + (size_t)c * kNB);
+    // LDS fronts in size buckets, so that small fronts get small LDS
+    // allocations and several workgroups per CU
+    for (int d0 = H.lds_ptr[l]; d0 < H.lds_ptr[l + 1];) {
+      const int c0 = H.cap[H.lds_nodes[d0]];
+      int bucket = 32;
+      while (bucket < c0) bucket += 16;
+      int d1 = d0;
+      while (d1 < H.lds_ptr[l + 1] && H.cap[H.lds_nodes[d1]] <= bucket) d1++;
+      const size_t bytes = lds_front_bytes(bucket, sizeof(T));
       ProfScope ps_(H, st, KC_FRONT_LDS);
       hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(d1 - d0), dim3(kTPB), bytes, st, P,
                          H.d_lds_nodes.p + d0);
+      d0 = d1;
     }
     const int s0 = H.small_ptr[l], s1 = H.small_ptr[l + 1];
     if (s1 > s0) {
@@ -1707,11 +1769,14 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
                          H.d_small_nodes.p + s0);
     }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
+    int maxcap_large = 0;
+    for (int q = g0; q < g1; q++) maxcap_large = std::max(maxcap_large, H.cap[H.large_nodes[q]]);
     for (int t = 0; t < H.large_panels[l]; t++) {
       {
         ProfScope ps_(H, st, KC_PANEL);
-        hipLaunchKernelGGL((k_panel<kPanelThreads, T>), dim3(g1 - g0), dim3(kPanelThreads), 0, st,
-                           P, H.d_large_nodes.p + g0);
+        hipLaunchKernelGGL((k_panel<kPanelThreads, T>), dim3(g1 - g0), dim3(kPanelThreads),
+                           sizeof(int) * 2 * (size_t)maxcap_large, st, P,
+                           H.d_large_nodes.p + g0);
       }
       const int slot = H.upd_lvl[l] + t;
       const int u0 = H.upd_ptr[slot], u1 = H.upd_ptr[slot + 1];
@@ -1739,7 +1804,7 @@ static void ensure_device(Handle& H) {
   HIPCHECK(hipGetDevice(&dev));
   HIPCHECK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   int mlds = kLdsFront;
-  while (mlds > 8 && (double)8 * ((double)mlds * mlds + (double)mlds * kNB) + 4096.0 > lim) mlds--;
+  while (mlds > 8 && (double)lds_front_bytes(mlds, 8) + 4096.0 > lim) mlds--;
   if (mlds != H.lds_front) {
     H.lds_front = mlds;
     std::vector<int> slack(H.N);
@@ -1876,21 +1941,30 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
   if (refine) {
     H.d_b.alloc(tot);
     H.d_r.alloc(tot);
-    H.d_norm.alloc(2 * (size_t)nrhs);
+    H.d_norm.alloc(2 * (size_t)nrhs + (size_t)kNormBlocks * nrhs);
     HIPCHECK(hipMemcpyAsync(H.d_b.p, dx, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
     std::vector<double> bn(nrhs), rn(nrhs);
-    hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(kTPB), 0, st, H.n, H.d_b.p, ldx, H.d_norm.p);
+    double* part = H.d_norm.p + 2 * (size_t)nrhs;
+    auto colnorm2 = [&](const double* v, double* out) {
+      hipLaunchKernelGGL(k_colnorm2_part, dim3(kNormBlocks, nrhs), dim3(kTPB), 0, st, H.n, v,
+                         ldx, part);
+      hipLaunchKernelGGL(k_colnorm2_final, dim3((nrhs + 63) / 64), dim3(64), 0, st, nrhs,
+                         (const double*)part, out);
+    };
+    colnorm2(H.d_b.p, H.d_norm.p);
     HIPCHECK(hipMemcpyAsync(bn.data(), H.d_norm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     run_solve_levels(H, dx, ldx, nrhs, st);
     int it = 0;
     double worst = 0.0;
     const dim3 rg((unsigned)((H.n * 16 + kTPB - 1) / kTPB));
     for (;; it++) {
-      ProfScope ps_(H, st, KC_REFINE);
-      hipLaunchKernelGGL(k_residual, rg, dim3(kTPB), 0, st, H.n, H.d_rowp.p, H.d_cols.p,
-                         H.d_vals.p, H.has_diag ? H.d_diag.p : nullptr, dx, H.d_b.p,
-                         H.d_r.p, ldx, nrhs);
-      hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(kTPB), 0, st, H.n, H.d_r.p, ldx, H.d_norm.p);
+      {
+        ProfScope ps_(H, st, KC_REFINE);
+        hipLaunchKernelGGL(k_residual, rg, dim3(kTPB), 0, st, H.n, H.d_rowp.p, H.d_cols.p,
+                           H.d_vals.p, H.has_diag ? H.d_diag.p : nullptr, dx, H.d_b.p,
+                           H.d_r.p, ldx, nrhs);
+      }
+      colnorm2(H.d_r.p, H.d_norm.p);
       HIPCHECK(hipMemcpyAsync(rn.data(), H.d_norm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
       worst = 0.0;
