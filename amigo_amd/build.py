@@ -2,7 +2,7 @@
 
     python -m amigo_amd.build
 
-Compiles amigo_amd/csrc/{symbolic.cpp,kkt_ldl.hip} with hipcc into
+Compiles amigo_amd/csrc/{symbolic.cpp,kkt_ldl.hip,assembly.hip} with hipcc into
 amigo_amd/libkktldl.so.  Every HIP translation unit is built with packed
 FP32 arithmetic disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
@@ -34,7 +34,7 @@ def build(verbose=True):
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     objs = []
-    for src in ("symbolic.cpp", "kkt_ldl.hip"):
+    for src in ("symbolic.cpp", "kkt_ldl.hip", "assembly.hip"):
         obj = os.path.join(build_dir, src + ".o")
         extra = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])

@@ -231,6 +231,39 @@ def poisson2d(N=256, seed=0, alpha=1e-4, mu=1e-6, delta_c=0.0):
                        dict(N=N, alpha=alpha, n_fixed=len(fixed), yd=yd))
 
 
+def poisson2d_elements(N=256, alpha=1e-4):
+    """The element view of :func:`poisson2d`'s KKT values: one component
+    group of N^2 Q1 elements, each with the 12 variables
+    (y_0..y_3, u_0..u_3, lambda_0..lambda_3) of its 4 nodes and the 12 x 12
+    Hessian of its Lagrangian term
+    [[M_e, 0, K_e^T], [0, alpha M_e, -M_e], [K_e, -M_e, 0]].
+
+    Returns dict(n, rowp, cols, idx (N^2 x 12), H (N^2 x 12 x 12), fixed):
+    assembling H into (rowp, cols) with the boundary y and lambda fixed gives
+    poisson2d(N).vals (up to the order of floating-point additions)."""
+    P = poisson2d(N=N, alpha=alpha)
+    h = 1.0 / N
+    Me, Ke = _q1_2d()
+    Me = Me * h * h
+    ii, jj = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    b = (ii * (N + 1) + jj).ravel()
+    conn = np.stack([b, b + (N + 1), b + (N + 1) + 1, b + 1], axis=1)
+    idx = np.concatenate([3 * conn, 3 * conn + 1, 3 * conn + 2], axis=1).astype(np.int32)
+    He = np.zeros((12, 12))
+    He[0:4, 0:4] = Me
+    He[4:8, 4:8] = alpha * Me
+    He[8:12, 0:4] = Ke
+    He[0:4, 8:12] = Ke.T
+    He[8:12, 4:8] = -Me
+    He[4:8, 8:12] = -Me.T
+    H = np.broadcast_to(He, (conn.shape[0], 12, 12)).copy()
+    nn = (N + 1) ** 2
+    gi, gj = np.divmod(np.arange(nn), N + 1)
+    bnd = np.nonzero((gi == 0) | (gj == 0) | (gi == N) | (gj == N))[0]
+    fixed = np.sort(np.concatenate([3 * bnd, 3 * bnd + 2])).astype(np.int32)
+    return dict(n=P.n, rowp=P.rowp, cols=P.cols, idx=idx, H=H, fixed=fixed, vals=P.vals)
+
+
 def _cartpole_derivs(q, u, w, g=9.81, L0=0.5, m1=1.0, m2=0.3):
     """f(q,u), df/d(q,u) and the Hessian of w^T f for every node (torch)."""
     import torch
@@ -401,6 +434,61 @@ def elasticity3d(N=40, seed=0, alpha=1e-4, mu=1e-6, delta_c=0.0):
     primal = np.sort(np.concatenate([yidx, uidx]))
     return KKTSnapshot("elasticity3d", n, rowp, cols, vals, diag, rhs, primal,
                        lidx[~isfix[lidx]], dict(N=N, alpha=alpha))
+
+
+def _bilinear(N, nc):
+    """(N+1)^2 x nc^2 bilinear interpolation from an nc x nc node grid on
+    [0,1]^2 to the (N+1)^2 mesh nodes (row-major in both)."""
+    t = np.linspace(0.0, 1.0, N + 1)
+    s = t * (nc - 1)
+    i0 = np.minimum(np.floor(s).astype(int), nc - 2)
+    w = s - i0
+    rows, cols, vals = [], [], []
+    for a in range(N + 1):
+        for da, wa in ((0, 1.0 - w[a]), (1, w[a])):
+            for db in (0, 1):
+                b = np.arange(N + 1)
+                wb = np.where(db == 0, 1.0 - w, w)
+                rows.append(a * (N + 1) + b)
+                cols.append((i0[a] + da) * nc + i0 + db)
+                vals.append(wa * wb)
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=((N + 1) ** 2, nc * nc))
+
+
+def poisson2d_sensitivity(N=256, ncoarse=32, nout=64, seed=0, **kw):
+    """Post-optimality sensitivity workload on :func:`poisson2d`: the target
+    field is y_d = B p with p the ncoarse^2 coefficients of a bilinear field
+    on a coarse grid; outputs J_k(z) = c_k^T z with c_k ~ N(0, 1).
+
+    Returns (snapshot, dict) where the dict holds B, the y-row map, the
+    gradient Jacobian dg/dp (n x ncoarse^2, sparse), the KKT right-hand side
+    map r(p) = -(dg/dp) p, the outputs C (nout x n) and p0 (the Gaussian
+    target sampled on the coarse grid)."""
+    P = poisson2d(N=N, seed=seed, **kw)
+    nn = (N + 1) ** 2
+    h = 1.0 / N
+    Me, _ = _q1_2d()
+    Me = Me * h * h
+    ii, jj = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    b = (ii * (N + 1) + jj).ravel()
+    conn = np.stack([b, b + (N + 1), b + (N + 1) + 1, b + 1], axis=1)
+    M = sp.coo_matrix((np.tile(Me.ravel(), conn.shape[0]),
+                       (np.repeat(conn, 4, axis=1).ravel(), np.tile(conn, (1, 4)).ravel())),
+                      shape=(nn, nn)).tocsr()
+    B = _bilinear(N, ncoarse)
+    gi, gj = np.divmod(np.arange(nn), N + 1)
+    free = ~((gi == 0) | (gj == 0) | (gi == N) | (gj == N))
+    MB = (sp.diags(free.astype(float)) @ M @ B).tocsr()        # free y rows only
+    Py = sp.csr_matrix((np.ones(nn), (3 * np.arange(nn), np.arange(nn))), shape=(P.n, nn))
+    dgdp = -(Py @ MB).tocsr()                                   # g_y = M (y - B p) + ...
+    rng = np.random.default_rng(seed + 1)
+    C = rng.standard_normal((nout, P.n))
+    tc = np.linspace(0, 1, ncoarse)
+    Xc, Yc = np.meshgrid(tc, tc, indexing="ij")
+    yd_full, _ = poisson_targets(N, seed)
+    p0 = (B.T @ yd_full) / np.maximum(B.T @ np.ones(nn), 1e-300)
+    return P, dict(B=B, dgdp=dgdp, C=C, p0=p0, ncoarse=ncoarse)
 
 
 GENERATORS = {

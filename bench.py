@@ -154,6 +154,57 @@ def roofline(r, P):
                 algorithmic_per_launch=byt / max(nl, 1))
 
 
+def bench_assembly(args, torch, reps=50, cpu=True):
+    """KKT value assembly of the same KKT from its Q1 elements (N^2 elements
+    x 12 variables, 144 contributions each) with the boundary rows fixed:
+    device time of k_gather_sum over reps launches (HIP events on torch's
+    current stream, the one the launch uses), and the reference's own host
+    assembly (CSRMat::add_row per element column + zero_rows_and_columns)
+    on the same contributions."""
+    from amigo_amd import workloads as W
+    from amigo_amd.assembly import KKTAssembler, element_contributions, element_values
+
+    E = W.poisson2d_elements(N=args.mesh)
+    A = KKTAssembler.from_elements(E["n"], E["rowp"], E["cols"], [E["idx"]], E["fixed"])
+    cvh = element_values(E["H"])
+    cv = torch.tensor(cvh, device="cuda")
+    out = torch.empty(A.nnz, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        A.values(cv, out=out)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        A.values(cv, out=out)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    kept = A.ncontrib - A.ndropped
+    # per kept contribution: 8 B value + 4 B gather index; per CSR entry:
+    # 4 B segment pointer + 1 B mode + 8 B value written
+    byt = 12.0 * kept + 13.0 * A.nnz + 4.0
+    ach = byt / (ms * 1e-3) / 1e9
+    res = dict(kernel="k_gather_sum", ms_per_launch=round(ms, 5), launches=reps,
+               ncontrib=int(A.ncontrib), ndropped=int(A.ndropped), nnz=int(A.nnz),
+               algorithmic_bytes=byt, achieved_GBs=round(ach, 1),
+               frac=round(ach / HBM_PEAK_GBS, 4),
+               bitwise_vs_snapshot=bool(np.array_equal(out.cpu().numpy(), E["vals"])))
+    if cpu:
+        from oracle import pyoracle as po
+        if po.ref_available():
+            r, c = element_contributions(E["idx"])
+            po.ref_assemble(E["n"], E["rowp"], E["cols"], r, c, cvh, E["fixed"], ncomp=12)
+            tc = []
+            for _ in range(3):
+                t = time.perf_counter()
+                po.ref_assemble(E["n"], E["rowp"], E["cols"], r, c, cvh, E["fixed"], ncomp=12)
+                tc.append((time.perf_counter() - t) * 1e3)
+            res["cpu_reference_ms"] = round(float(np.median(tc)), 3)
+            res["cpu_reference_sample"] = ("reference host assembly (serial add_row + fixed "
+                                           "rows), median of 3, 1 core")
+    return res
+
+
 def cpu_baseline(P, perm, reps):
     """The reference's SparseLDL (oracle/_ref, compiled from its own headers)
     on the same KKT with the same elimination order (the matrix permuted by
@@ -213,6 +264,7 @@ def main():
             raise
     head = "fp32" if "fp32" in results else "fp64"
     r = results[head]
+    asm = bench_assembly(args, torch, cpu=not args.no_cpu_baseline)
     perm = r["s"].get_perm()[0]
     cpu = None if args.no_cpu_baseline else cpu_baseline(P, perm, args.cpu_reps)
     out = {
@@ -246,6 +298,7 @@ def main():
         },
         "kernels": {k: {"ms": round(t, 3), "launches": n} for k, (t, n) in r["kt"].items()},
         "roofline": roofline(r, P),
+        "assembly": asm,
         "cpu_baseline": cpu,
     }
     if cpu and cpu.get("inertia"):

@@ -67,12 +67,38 @@ def _load(path, prefix):
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int]
         lib.ref_analyze.restype = vp
+        lib.ref_assemble.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.c_int, _i32p, _i32p,
+                                     _i32p, _f64p, ctypes.c_int, _i32p, _f64p]
     _libs[path] = lib
     return lib
 
 
 def ref_available():
     return os.path.exists(REF_SO)
+
+
+def ref_assemble(n, rowp, cols, crow, ccol, cvals, fixed=(), ncomp=None):
+    """The reference's own host assembly (CSRMat::add_row per call, then
+    FixedVariables::zero_rows_and_columns).  Contributions are grouped into
+    add_row calls of ncomp consecutive entries sharing a row (one call per
+    element column, as the reference makes them); ncomp=None makes one call
+    per contribution."""
+    lib = _load(REF_SO, "ref_")
+    crow = np.ascontiguousarray(crow, np.int32)
+    ccol = np.ascontiguousarray(ccol, np.int32)
+    cv = np.ascontiguousarray(cvals, np.float64)
+    nc = 1 if ncomp is None else int(ncomp)
+    assert len(crow) % nc == 0
+    ncalls = len(crow) // nc
+    call_row = np.ascontiguousarray(crow[::nc])
+    assert np.all(crow.reshape(ncalls, nc) == call_row[:, None])
+    call_ptr = np.arange(0, len(crow) + 1, nc, dtype=np.int32)
+    fx = np.ascontiguousarray(fixed, np.int32)
+    out = np.zeros(int(rowp[n]))
+    lib.ref_assemble(n, _ip(np.ascontiguousarray(rowp, np.int32)),
+                     _ip(np.ascontiguousarray(cols, np.int32)), ncalls, _ip(call_row),
+                     _ip(call_ptr), _ip(ccol), _dp(cv), len(fx), _ip(fx), _dp(out))
+    return out
 
 
 class _Base:

@@ -27,6 +27,7 @@
 #include <omp.h>
 #define private public
 #include "sparse_ldl.h"
+#include "fixed_variables.h"
 #undef private
 
 using Ldl = amigo::SparseLDL<double>;
@@ -127,6 +128,29 @@ void ref_node_info(void* hh, int* npiv, int* ndel, int* n2x2, int* root) {
     }
     n2x2[k] = c;
   }
+}
+
+// KKT value assembly as OptimizationProblem::hessian does it on the host
+// (include/optimization_problem.h:1063-1087): zero the values, one
+// CSRMat::add_row per (element, column) call, then
+// FixedVariables::zero_rows_and_columns.  Call c adds call_vals[p] at
+// (call_row[c], call_cols[p]) for p in [call_ptr[c], call_ptr[c+1]).
+void ref_assemble(int n, const int* rowp, const int* cols, int ncalls,
+                  const int* call_row, const int* call_ptr, const int* call_cols,
+                  const double* call_vals, int nfixed, const int* fixed,
+                  double* vals_out) {
+  auto mat = amigo::CSRMat<double>::create_from_csr_data(n, n, rowp[n], rowp, cols);
+  mat->zero();
+  for (int c = 0; c < ncalls; c++)
+    mat->add_row(call_row[c], call_ptr[c + 1] - call_ptr[c], call_cols + call_ptr[c],
+                 call_vals + call_ptr[c]);
+  if (nfixed > 0) {
+    auto dofs = std::make_shared<amigo::Vector<int>>(nfixed);
+    std::copy(fixed, fixed + nfixed, dofs->get_array());
+    amigo::FixedVariables fv(dofs, mat);
+    fv.zero_rows_and_columns<amigo::ExecPolicy::SERIAL>(mat);
+  }
+  std::copy(mat->get_data_ptr(), mat->get_data_ptr() + rowp[n], vals_out);
 }
 
 }  // extern "C"

@@ -28,6 +28,17 @@ def test_every_declared_symbol_is_exported():
     assert sorted(L.EXPORTS) == decl
 
 
+def test_every_assembly_symbol_is_exported():
+    from amigo_amd import assembly as A
+    lib = L.load_library()
+    text = open(os.path.join(ROOT, "include", "kkt_asm.h")).read()
+    decl = sorted(set(re.findall(r"\b(kkt_asm_[a-z_]+)\s*\(", text)))
+    assert len(decl) == 5
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert sorted(A.EXPORTS) == decl
+
+
 def test_default_options_match_amigo_solver():
     lib = L.load_library()
     o = L.Options()
