@@ -949,6 +949,539 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
   }
 }
 
+// --------------------------------------------------------------------------
+// Large-front panel with the panel's L held in LDS (fronts up to R * NT rows).
+//
+// The panel factorization of a large front runs on one workgroup of NT
+// threads; thread t owns rows kstart + t + j * NT (j < R).  The L entries of
+// the panel columns live in LDS for the whole panel, so updating front
+// column k by the panel so far, W[i, kw] = F[i, k] - sum_c L[i, c] W[k, c],
+// reads only the column from HBM (prefetched two columns ahead into
+// registers).  The W entries of the positions that can become pivot rows in
+// this panel (kstart .. kstart + 17) are cached in LDS by their owners in the
+// same barrier interval as the column reduction, and every thread reads the
+// pivot row from there.  W is written to HBM as it is produced (the trailing
+// update reads it) and L once, at the end of the panel.  The barriers inside
+// a pivot step order LDS only, so HBM stores and prefetch loads stay in
+// flight across them.  A 1x1 pivot costs one barrier and a 2x2 pivot whose
+// partner is the next candidate (imax == k+1, the common case in KKT fronts)
+// two; pivots that need a symmetric swap, and delays, take a slower path
+// that swaps in HBM and reloads the prefetch window.  The pivot decisions are
+// panel_factor's (mode 0 threshold with delays, mode 1 Bunch-Kaufman) on the
+// same FP64 values; L entries are formed with explicit FMAs.
+// --------------------------------------------------------------------------
+constexpr int kWCRows = kNBLarge + 2;
+
+template <int NW>
+struct RRShared {
+  double pb[2][NW];  // column reduction: best candidate |v|
+  int pi[2][NW];     //   its row
+  double pr[2][NW];  //   max |v| over the rows below the candidates
+  double qr[2][NW];  // row reduction of the imax column
+  double wkk[2], fk1k[2], wik[2], wii[2];
+  double wc[kWCRows][kNBLarge];  // W rows of positions kstart .. kstart + kWCRows - 1
+  double xw[kNBLarge];           // W row of a general imax
+  double xa[kNBLarge + 2], xb[kNBLarge + 2];
+  double swii, swik;
+  int cT[kNBLarge];  // per panel column: 1 1x1, 2/3 first/second of 2x2, 4 unscaled
+  int npos, nneg, n2;
+};
+
+// Workgroup barrier that orders LDS only: it waits for this wave's LDS
+// operations, not for its HBM loads and stores (a full __syncthreads()
+// waits for every outstanding global access, which would stall each pivot
+// step on the W stores and on the prefetch loads in flight).
+__device__ inline void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Cross-lane moves by DPP (VALU speed): quad_perm [1,0,3,2] and [2,3,0,1],
+// row_half_mirror and row_mirror pair every lane of a 16-lane row with every
+// other after four steps.
+template <int CTRL>
+__device__ inline int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = dpp_i<CTRL>((int)b);
+  const int hi = dpp_i<CTRL>((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double lane_d(double x, int l) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// (max |v| with the first row on ties, max of a second value); the order of
+// combination does not change the result.
+__device__ inline void am_combine(double& b, int& i, double& r, double ob, int oi, double orr) {
+  if (ob > b || (ob == b && oi < i)) {
+    b = ob;
+    i = oi;
+  }
+  r = fmax(r, orr);
+}
+template <int CTRL>
+__device__ inline void am_step(double& b, int& i, double& r) {
+  am_combine(b, i, r, dpp_d<CTRL>(b), dpp_i<CTRL>(i), dpp_d<CTRL>(r));
+}
+__device__ inline void row_argmax2(double& b, int& i, double& r) {
+  am_step<0xB1>(b, i, r);
+  am_step<0x4E>(b, i, r);
+  am_step<0x141>(b, i, r);
+  am_step<0x140>(b, i, r);
+}
+// wave-wide result, uniform in every lane
+__device__ inline void wave_argmax2(double& b, int& i, double& r) {
+  row_argmax2(b, i, r);
+  double rb = lane_d(b, 0), rr = lane_d(r, 0);
+  int ri = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+  for (int q = 1; q < kWave / 16; q++)
+    am_combine(rb, ri, rr, lane_d(b, 16 * q), __builtin_amdgcn_readlane(i, 16 * q),
+               lane_d(r, 16 * q));
+  b = rb;
+  i = ri;
+  r = rr;
+}
+__device__ inline double row_max(double x) {
+  x = fmax(x, dpp_d<0xB1>(x));
+  x = fmax(x, dpp_d<0x4E>(x));
+  x = fmax(x, dpp_d<0x141>(x));
+  x = fmax(x, dpp_d<0x140>(x));
+  return x;
+}
+__device__ inline double wave_max(double x) {
+  x = row_max(x);
+  double r = lane_d(x, 0);
+#pragma unroll
+  for (int q = 1; q < kWave / 16; q++) r = fmax(r, lane_d(x, 16 * q));
+  return r;
+}
+
+template <int NT, int R, class T>
+__global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
+  constexpr int NB = kNBLarge;
+  constexpr int NW = NT / kWave;
+  constexpr int RS = R * NT;        // rows of a panel column in LDS
+  extern __shared__ double s_rr[];  // Ls[NB][RS] of T, then vars[m], pre[m]
+  __shared__ RRShared<NW> sh;
+  __shared__ PanelState s_st;
+  const int p = nodes[blockIdx.x];
+  if (P.stat[p]) return;
+  const int tid = threadIdx.x;
+  const int wv = tid / kWave, lane = tid % kWave;
+  if (tid == 0) {
+    s_st = P.ps[p];
+    sh.npos = s_st.npos;
+    sh.nneg = s_st.nneg;
+    sh.n2 = s_st.n2;
+  }
+  __syncthreads();
+  int k = s_st.k, nc = s_st.nc, kw = 0;
+  const int kstart = k;
+  if (k < nc) {
+    const int m = P.m[p];
+    const int mode = P.mode[p];
+    const int nb = max(2, min(NB, m));
+    const long long io = P.ioff[p];
+    T* F = front_ptr<T>(P, p);
+    const int ld = m;
+    double* Wg = P.W + P.woff[p];
+    const int ldw = P.cap[p];
+    const double ustab = P.ustab, ptol = P.ptol;
+    const double alpha = 0.6403882032022076;  // (1 + sqrt(17)) / 8
+    T* Ls = (T*)s_rr;
+    int* vars = (int*)(Ls + (size_t)NB * RS);
+    int* pre = vars + m;
+    for (int i = k + tid; i < m; i += NT) {
+      vars[i] = P.vars[io + i];
+      pre[i] = P.pre[io + i];
+    }
+    auto rowof = [&](int j) { return kstart + tid + j * NT; };
+    auto lsx = [&](int c, int j) -> T& { return Ls[(size_t)c * RS + j * NT + tid]; };
+    auto lsr = [&](int c, int r) -> T& { return Ls[(size_t)c * RS + (r - kstart)]; };
+    // front columns k .. k+3 of the owned rows: c0, c1 ready, f2, f3 in
+    // flight (loads are unconditional, rows and columns clamped into the
+    // front, so that the compiler waits only where a value is first used)
+    T c0[R], c1[R], f2[R], f3[R];
+    auto ld_col = [&](int col, T (&t)[R]) {
+      const size_t co = (size_t)min(col, m - 1) * ld;
+#pragma unroll
+      for (int j = 0; j < R; j++) t[j] = F[min(rowof(j), m - 1) + co];
+    };
+    auto reload = [&]() {
+      ld_col(k, c0);
+      ld_col(k + 1, c1);
+      ld_col(k + 2, f2);
+      ld_col(k + 3, f3);
+    };
+    reload();
+    __syncthreads();
+    int it = 0;
+    while (kw < nb - 1 && k < nc) {
+      const int par = it & 1;
+      // W[:, kw]: column k updated by the panel so far
+      const double* wr = sh.wc[k - kstart];
+      double v[R], vp[R];
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        v[j] = (double)c0[j];
+        vp[j] = 0.0;
+      }
+      for (int c = 0; c < kw; c++) {
+        const double w = wr[c];
+#pragma unroll
+        for (int j = 0; j < R; j++) v[j] = __builtin_fma(-(double)lsx(c, j), w, v[j]);
+      }
+      double lbest = -1.0, lrest = 0.0;
+      int lidx = 0x7fffffff;
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int r = rowof(j);
+        if (r >= k && r < m) {
+          const double x = v[j];
+          const double a = fabs(x);
+          if (r > k && r < nc) {
+            if (a > lbest) {
+              lbest = a;
+              lidx = r;
+            }
+          } else if (r >= nc) {
+            lrest = fmax(lrest, a);
+          }
+          if (r == k) sh.wkk[par] = x;
+          if (r == k + 1) {
+            sh.fk1k[par] = (double)c0[j];
+            sh.wik[par] = x;
+          }
+          if (r - kstart < kWCRows) sh.wc[r - kstart][kw] = x;
+        }
+      }
+      wave_argmax2(lbest, lidx, lrest);
+      if (lane == 0) {
+        sh.pb[par][wv] = lbest;
+        sh.pi[par][wv] = lidx;
+        sh.pr[par][wv] = lrest;
+      }
+      lds_sync();
+      {
+        double bb = -1.0, rr = 0.0;
+        int ii = 0x7fffffff;
+        if (lane < NW) {
+          bb = sh.pb[par][lane];
+          ii = sh.pi[par][lane];
+          rr = sh.pr[par][lane];
+        }
+        row_argmax2(bb, ii, rr);
+        lbest = lane_d(bb, 0);
+        lidx = __builtin_amdgcn_readlane(ii, 0);
+        lrest = lane_d(rr, 0);
+      }
+      const double wkk = sh.wkk[par];
+      const double awkk = fabs(wkk);
+      int imax = 0;
+      double awik = 0.0;
+      if (k + 1 < nc) {
+        imax = lidx;
+        awik = lbest;
+      }
+      const double colmax = fmax(awik, lrest);
+      int step = 0, kpiv = -1;
+      bool zero_col = false, need_row = false;
+      if (mode == 1) {
+        if (fmax(awkk, colmax) == 0.0 || isnan(awkk)) {
+          step = 1;
+          kpiv = k;
+          zero_col = true;
+        } else if (awkk >= alpha * colmax) {
+          step = 1;
+          kpiv = k;
+        } else {
+          need_row = true;
+        }
+      } else {
+        if (awkk >= ptol && awkk >= ustab * colmax) {
+          step = 1;
+          kpiv = k;
+        } else if (k + 1 < nc) {
+          need_row = true;
+        }
+      }
+      double wii = 0.0, wik = 0.0;
+      if (need_row) {
+        double lrow = 0.0;
+        const double* wr2;
+        if (imax == k + 1) {
+          // the imax column is the prefetched column k+1; its entry in row
+          // k is F[k+1, k], published with the column reduction
+          wr2 = sh.wc[k + 1 - kstart];
+#pragma unroll
+          for (int j = 0; j < R; j++) vp[j] = (rowof(j) == k) ? sh.fk1k[par] : (double)c1[j];
+        } else {
+          // general imax: its W row from HBM (written by its owner), its
+          // column (row imax left of the diagonal) from the front
+#pragma unroll
+          for (int j = 0; j < R; j++)
+            if (rowof(j) == imax) {
+              for (int c = 0; c < kw; c++) sh.xw[c] = Wg[imax + (size_t)c * ldw];
+              sh.swik = v[j];
+            }
+#pragma unroll
+          for (int j = 0; j < R; j++) {
+            const int r = min(rowof(j), m - 1);
+            vp[j] = (double)(r < imax ? F[imax + (size_t)r * ld] : F[r + (size_t)imax * ld]);
+          }
+          lds_sync();
+          wr2 = sh.xw;
+        }
+        for (int c = 0; c < kw; c++) {
+          const double w = wr2[c];
+#pragma unroll
+          for (int j = 0; j < R; j++) vp[j] = __builtin_fma(-(double)lsx(c, j), w, vp[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          const int r = rowof(j);
+          if (r >= k && r < m) {
+            const double x = vp[j];
+            if (r != imax) lrow = fmax(lrow, fabs(x));
+            if (r == imax) sh.wii[par] = x;
+            if (imax == k + 1 && r - kstart < kWCRows) sh.wc[r - kstart][kw + 1] = x;
+          }
+        }
+        lrow = wave_max(lrow);
+        if (lane == 0) sh.qr[par][wv] = lrow;
+        lds_sync();
+        wii = sh.wii[par];
+        wik = (imax == k + 1) ? sh.wik[par] : sh.swik;
+        const double rowmax = lane_d(row_max(lane < NW ? sh.qr[par][lane] : 0.0), 0);
+        const double awii = fabs(wii);
+        if (mode == 1) {
+          if (awkk >= alpha * colmax * (colmax / rowmax)) {
+            step = 1;
+            kpiv = k;
+          } else if (awii >= alpha * rowmax) {
+            step = 1;
+            kpiv = imax;
+          } else {
+            step = 2;
+            kpiv = imax;
+          }
+        } else {
+          if (awii >= ptol && awii >= ustab * rowmax) {
+            step = 1;
+            kpiv = imax;
+          } else {
+            const double det = fabs(wkk * wii - wik * wik);
+            if (det >= ptol) {
+              const double delta = det / fmax(awii + awik, awkk + awik);
+              if (delta >= ustab * colmax && delta >= ustab * rowmax) {
+                step = 2;
+                kpiv = imax;
+              }
+            }
+          }
+        }
+      }
+      // the decision is the same in every lane; keep it in scalar registers
+      step = __builtin_amdgcn_readfirstlane(step);
+      kpiv = __builtin_amdgcn_readfirstlane(kpiv);
+      const bool fast = step > 0 && kpiv == k + step - 1;
+      if (!fast) {
+        // delay (step 0), 1x1 at imax, or 2x2 whose partner is not k+1
+        if (step == 1) {
+#pragma unroll
+          for (int j = 0; j < R; j++) v[j] = vp[j];  // the pivot column is imax
+        }
+        if (step > 0) {
+#pragma unroll
+          for (int j = 0; j < R; j++) {
+            const int r = rowof(j);
+            if (r >= k && r < m) {
+              Wg[r + (size_t)kw * ldw] = v[j];
+              if (step == 2) Wg[r + (size_t)(kw + 1) * ldw] = vp[j];
+            }
+          }
+        }
+        __syncthreads();
+        const int a = (step == 0) ? k : k + step - 1;
+        const int b = (step == 0) ? nc - 1 : kpiv;
+        const int cnt = kw + step;
+        if (a != b) {
+          for (int c = tid; c < cnt; c += NT) {
+            const double ta = Wg[a + (size_t)c * ldw];
+            Wg[a + (size_t)c * ldw] = Wg[b + (size_t)c * ldw];
+            Wg[b + (size_t)c * ldw] = ta;
+          }
+          for (int c = tid; c < kw; c += NT) {
+            const T tl = lsr(c, a);
+            lsr(c, a) = lsr(c, b);
+            lsr(c, b) = tl;
+          }
+          sym_swap<NT, T>(F, ld, m, vars, pre, a, b);
+#pragma unroll
+          for (int j = 0; j < R; j++) {
+            const int r = rowof(j);
+            if (r == a || r == b) {
+              double* dst = (r == a) ? sh.xa : sh.xb;
+              dst[0] = v[j];
+              dst[1] = vp[j];
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int j = 0; j < R; j++) {
+            const int r = rowof(j);
+            if (r == a || r == b) {
+              const double* src = (r == a) ? sh.xb : sh.xa;
+              v[j] = src[0];
+              vp[j] = src[1];
+            }
+          }
+          // cached W rows of the swapped positions, from HBM
+          for (int c = tid; c < 2 * cnt; c += NT) {
+            const int r = (c < cnt) ? a : b;
+            const int cc = (c < cnt) ? c : c - cnt;
+            if (r - kstart < kWCRows) sh.wc[r - kstart][cc] = Wg[r + (size_t)cc * ldw];
+          }
+        }
+        if (step == 0) nc -= 1;
+      }
+      if (step == 1) {
+        const double d = (kpiv == k) ? wkk : wii;
+        const double dinv = (zero_col || d == 0.0) ? 0.0 : 1.0 / d;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          const int r = rowof(j);
+          if (r >= k && r < m) {
+            lsx(kw, j) = zero_col ? (T)v[j] : (T)(v[j] * dinv);
+            if (fast) Wg[r + (size_t)kw * ldw] = v[j];
+            else if (r - kstart < kWCRows) sh.wc[r - kstart][kw] = v[j];
+          }
+        }
+        if (lane == 0) sh.cT[kw] = zero_col ? 4 : 1;  // every wave writes the same value
+        if (tid == 0) {
+          F[k + (size_t)k * ld] = (T)d;
+          P.ptype[io + k] = 1;
+          P.dv[io + k] = d;
+          if (d >= 0.0) sh.npos++;
+          else sh.nneg++;
+        }
+      } else if (step == 2) {
+        const double d11 = wkk, d21 = wik, d22 = wii;
+        const double det = d11 * d22 - d21 * d21;
+        const double detinv = 1.0 / det;
+        const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          const int r = rowof(j);
+          if (r >= k && r < m) {
+            lsx(kw, j) = (T)__builtin_fma(v[j], i00, vp[j] * i01);
+            lsx(kw + 1, j) = (T)__builtin_fma(v[j], i01, vp[j] * i11);
+            if (fast) {
+              Wg[r + (size_t)kw * ldw] = v[j];
+              Wg[r + (size_t)(kw + 1) * ldw] = vp[j];
+            } else if (r - kstart < kWCRows) {
+              sh.wc[r - kstart][kw] = v[j];
+              sh.wc[r - kstart][kw + 1] = vp[j];
+            }
+          }
+        }
+        if (lane == 0) {
+          sh.cT[kw] = 2;
+          sh.cT[kw + 1] = 3;
+        }
+        if (tid == 0) {
+          F[k + (size_t)k * ld] = (T)d11;
+          F[k + 1 + (size_t)k * ld] = (T)d21;
+          F[k + 1 + (size_t)(k + 1) * ld] = (T)d22;
+          P.dv[io + k] = d11;
+          P.dv2[io + k] = d21;
+          P.dv[io + k + 1] = d22;
+          vars[k] = -(vars[k] + 1);
+          vars[k + 1] = -(vars[k + 1] + 1);
+          P.ptype[io + k] = 2;
+          P.ptype[io + k + 1] = 3;
+          if (det >= 0.0) {
+            if (d11 >= 0.0) sh.npos += 2;
+            else sh.nneg += 2;
+          } else {
+            sh.npos++;
+            sh.nneg++;
+          }
+          sh.n2++;
+        }
+      }
+      k += step;
+      kw += step;
+      if (!fast) {
+        // the swaps and the cached rows are complete for every thread
+        __syncthreads();
+        reload();
+      } else if (step == 1) {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          c0[j] = c1[j];
+          c1[j] = f2[j];
+          f2[j] = f3[j];
+        }
+        ld_col(k + 3, f3);
+      } else {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          c0[j] = f2[j];
+          c1[j] = f3[j];
+        }
+        ld_col(k + 2, f2);
+        ld_col(k + 3, f3);
+      }
+      it++;
+    }
+    __syncthreads();
+    // L of the panel below each pivot block (the D entries were written at
+    // the pivot)
+    for (int c = 0; c < kw; c++) {
+      const int pos = kstart + c;
+      const int last = (sh.cT[c] == 2) ? pos + 1 : pos;
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int r = rowof(j);
+        if (r > last && r < m) F[r + (size_t)pos * ld] = lsx(c, j);
+      }
+    }
+    for (int i = kstart + tid; i < m; i += NT) {
+      P.vars[io + i] = vars[i];
+      P.pre[io + i] = pre[i];
+    }
+    if (tid == 0 && k >= nc) {
+      const int fs = P.ns[p] + P.din[p];
+      P.npiv[p] = nc;
+      P.ndel[p] = fs - nc;
+      P.cnt[3 * p] = sh.npos;
+      P.cnt[3 * p + 1] = sh.nneg;
+      P.cnt[3 * p + 2] = sh.n2;
+    }
+  }
+  if (tid == 0) {
+    PanelState st = s_st;
+    st.k = k;
+    st.nc = nc;
+    st.kstart = kstart;
+    st.kw = kw;
+    st.npos = sh.npos;
+    st.nneg = sh.nneg;
+    st.n2 = sh.n2;
+    P.ps[p] = st;
+  }
+}
+
 // FP32 fronts use v_mfma_f32_16x16x4f32 (FP32 accumulate), whose output is
 // D[4(l/16) + r][l%16] (also checked on gfx950).
 template <class T>
@@ -1731,6 +2264,30 @@ static size_t lds_front_bytes(int c, size_t esize) {
          3 * sizeof(int) * (size_t)c + 16;
 }
 
+// LDS-resident panel for a level whose largest front slot has maxcap rows:
+// 512 threads with R = 1, 2 or 4 rows each.  False (use the global-memory
+// panel) when the panel's L does not fit LDS: above 2048 rows for FP32
+// fronts, above 1024 rows for FP64 fronts.
+template <class T>
+static bool launch_panel_rr(int maxcap, dim3 grid, hipStream_t st, const DevPlan& P,
+                            const int* nodes) {
+  constexpr int NT = 512;
+  auto lds = [&](int r) {
+    return sizeof(T) * kNBLarge * (size_t)r * NT + sizeof(int) * 2 * (size_t)maxcap;
+  };
+  if (maxcap <= NT) {
+    hipLaunchKernelGGL((k_panel_rr<NT, 1, T>), grid, dim3(NT), lds(1), st, P, nodes);
+  } else if (maxcap <= 2 * NT) {
+    hipLaunchKernelGGL((k_panel_rr<NT, 2, T>), grid, dim3(NT), lds(2), st, P, nodes);
+  } else if (maxcap <= 4 * NT && sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((k_panel_rr<NT, 4, T>), grid, dim3(NT), lds(4), st, P, nodes);
+  } else {
+    return false;
+  }
+  return true;
+}
+
 template <class T>
 static void run_factor_levels_t(Handle& H, hipStream_t st) {
   DevPlan P = dev_plan(H);
@@ -1774,9 +2331,10 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
     for (int t = 0; t < H.large_panels[l]; t++) {
       {
         ProfScope ps_(H, st, KC_PANEL);
-        hipLaunchKernelGGL((k_panel<kPanelThreads, T>), dim3(g1 - g0), dim3(kPanelThreads),
-                           sizeof(int) * 2 * (size_t)maxcap_large, st, P,
-                           H.d_large_nodes.p + g0);
+        if (!launch_panel_rr<T>(maxcap_large, dim3(g1 - g0), st, P, H.d_large_nodes.p + g0))
+          hipLaunchKernelGGL((k_panel<kPanelThreads, T>), dim3(g1 - g0), dim3(kPanelThreads),
+                             sizeof(int) * 2 * (size_t)maxcap_large, st, P,
+                             H.d_large_nodes.p + g0);
       }
       const int slot = H.upd_lvl[l] + t;
       const int u0 = H.upd_ptr[slot], u1 = H.upd_ptr[slot + 1];

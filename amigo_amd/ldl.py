@@ -112,6 +112,13 @@ def load_library(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise ImportError(f"{path} not built; run `python -m amigo_amd.build`")
+    # torch ships its own HIP runtime; loading it first makes the library
+    # bind to that same runtime instead of a second copy from /opt/rocm
+    # (two runtimes in one process do not see the device consistently)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     lib.kkt_ldl_default_options.argtypes = [ctypes.POINTER(Options)]
     lib.kkt_ldl_create.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.POINTER(Options),
@@ -203,6 +210,14 @@ class Vector:
 
     def copy(self, other):
         self.array[:] = _host_array(other)
+
+    def zero(self):
+        self.array[:] = 0.0
+
+    def add_scalar_at(self, indices, value):
+        """array[indices[i]] += value, one index at a time as the reference
+        does (include/vector.h:253-265), so repeated indices add repeatedly."""
+        np.add.at(self.array, np.asarray(_host_array(indices), dtype=np.intp), value)
 
     def copy_device_to_host(self):
         pass

@@ -41,13 +41,22 @@ def parse():
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32", "both"])
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only run 5 device-to-device copies of MiB MiB (the known-size "
+                         "kernel that scales PMC FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--pmc-probe", choices=["fp64", "fp32"], default=None,
+                    help="only run one factor + refined solve in this precision (the "
+                         "program the PMC traffic passes profile) and exit")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 --pmc passes behind roofline.traffic")
     return ap.parse_args()
 
 
-def large_front_work(s, P):
+def large_front_work(s, P, esize=8):
     """Algorithmic work of the large fronts (nominal dimension > 256), from the
     symbolic structure and the pivots of the last factorization:
-    bytes: each eliminated column read once and written once as L;
+    bytes: each eliminated column read once and written once as L (esize
+    bytes per front entry: 8 for FP64 fronts, 4 for FP32 fronts);
     flops: the rank-1 Schur updates of the lower triangle."""
     sn = s.get_supernodes()
     info = s.node_info()
@@ -67,16 +76,17 @@ def large_front_work(s, P):
     for k in np.nonzero(big)[0]:
         j = np.arange(info["npiv"][k], dtype=np.float64)
         rem = m[k] - j
-        byt += float(np.sum(16.0 * rem))
+        byt += float(np.sum(2.0 * esize * rem))
         flops += float(np.sum((rem - 1.0) * rem))
     return byt, flops
 
 
-def solve_bytes(s):
+def solve_bytes(s, esize=8):
     st = s.stats()
     n = st["n"]
-    # L read in each sweep + 8 B per solution entry read/written per sweep
-    return 2.0 * 8.0 * st["nnz_l"] + 2.0 * 2.0 * 8.0 * n
+    # L read in each sweep (esize bytes per entry) + 8 B per solution entry
+    # read and written per sweep
+    return 2.0 * esize * st["nnz_l"] + 2.0 * 2.0 * 8.0 * n
 
 
 def run_gpu(P, precision, args, torch):
@@ -120,7 +130,7 @@ def run_gpu(P, precision, args, torch):
     kt_s = s.kernel_times()
     s.set_profiling(False)
     kt = {k: (kt_f[k][0] + kt_s[k][0], kt_f[k][1] + kt_s[k][1]) for k in kt_f}
-    return dict(s=s, ms=ms, status=status, inertia=inertia, residual=res,
+    return dict(s=s, ms=ms, status=status, inertia=inertia, residual=res, precision=precision,
                 refine_steps=stats["refine_steps"], factor_ms=fac_ms,
                 solve_ms=sol_ms, t_analyze_s=t_analyze, stats=stats, kt=kt)
 
@@ -130,6 +140,7 @@ def roofline(r, P):
     dom = max(kt, key=lambda k: kt[k][0])
     t_ms, nl = kt[dom]
     s = r["s"]
+    esize = 4 if r["precision"] == "fp32" else 8
     if dom == "update":
         _, flops = large_front_work(s, P)
         ach = flops / (t_ms * 1e-3) / 1e12
@@ -138,20 +149,71 @@ def roofline(r, P):
                     launches=nl, avg_launch_us=t_ms * 1e3 / max(nl, 1),
                     algorithmic_per_launch=flops / max(nl, 1))
     if dom in ("forward", "backward"):
-        byt = solve_bytes(s) / 2.0 * (1 + r["refine_steps"])
+        byt = solve_bytes(s, esize) / 2.0 * (1 + r["refine_steps"])
         ach = byt / (t_ms * 1e-3) / 1e9
         return dict(kernel="k_" + dom, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None, launches=nl,
                     avg_launch_us=t_ms * 1e3 / max(nl, 1),
                     algorithmic_per_launch=byt / max(nl, 1))
-    byt, _ = large_front_work(s, P)
+    byt, _ = large_front_work(s, P, esize)
     if dom != "panel":
-        byt = 16.0 * s.stats()["nnz_l"]
+        byt = 2.0 * esize * s.stats()["nnz_l"]
     ach = byt / (t_ms * 1e-3) / 1e9
     return dict(kernel={"panel": "k_panel"}.get(dom, "k_" + dom), bound="hbm",
                 achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
                 traffic=None, launches=nl, avg_launch_us=t_ms * 1e3 / max(nl, 1),
                 algorithmic_per_launch=byt / max(nl, 1))
+
+
+KERNEL_NAMES = {"panel": "k_panel", "update": "k_update", "forward": "k_forward",
+                "backward": "k_backward", "factor_small": "k_factor",
+                "front_lds": "k_front_lds", "assemble": "k_assemble",
+                "prepare": "k_prepare", "residual": "k_residual"}
+
+
+def _pmc_pass(counter, child_args, timeout=180):
+    """Per-dispatch values of one PMC counter for this script run with
+    child_args under rocprofv3 (a child process), as {kernel name: [KiB]}."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        raise RuntimeError("rocprofv3 not found")
+    d = tempfile.mkdtemp(prefix="kkt_pmc_")
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__)] + child_args
+    subprocess.run(cmd, check=True, timeout=timeout, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL, env=dict(os.environ, TMPDIR="/tmp"), cwd="/tmp")
+    out = {}
+    with open(os.path.join(d, "run_counter_collection.csv")) as f:
+        for row in csv.DictReader(f):
+            out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def pmc_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE) in one
+    factor + refined solve of the headline precision, each counter scaled by
+    what it reads for device copies of known size (5 x 256 MiB)."""
+    copy_bytes = 256 << 20
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = _pmc_pass(counter, ["--copy-calibration", "256"])
+        cv = [v for k, vs in cal.items() if "copyBuffer" in k for v in vs]
+        if not cv:
+            raise RuntimeError("no copy kernel in the calibration pass")
+        scale = copy_bytes / 1024.0 / (sum(cv) / len(cv))
+        run = _pmc_pass(counter, ["--pmc-probe", precision, "--order", "nd"])
+        kv = [v for k, vs in run.items() if kernel + "<" in k or kernel + "(" in k for v in vs]
+        if not kv:
+            raise RuntimeError(f"no {kernel} dispatch in the PMC pass")
+        res[counter] = dict(kib_per_launch=sum(kv) / len(kv), scale=scale, launches=len(kv))
+    byt = sum(1024.0 * v["kib_per_launch"] * v["scale"] for v in res.values())
+    return byt, res
 
 
 def bench_assembly(args, torch, reps=50, cpu=True):
@@ -250,6 +312,25 @@ def main():
     from amigo_amd import workloads as W
 
     assert torch.cuda.is_available(), "needs an MI355X (cuda:0)"
+    if args.copy_calibration:
+        a = torch.ones(args.copy_calibration << 17, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": a.numel() * 8, "copies": 5}))
+        return
+    if args.pmc_probe:
+        from amigo_amd.ldl import FP32_REFINE, FP64, CSRMat, OrderingType, SparseLDL
+        P = W.poisson2d(N=args.mesh)
+        order = OrderingType.NESTED_DISSECTION if args.order == "nd" else OrderingType.AMD
+        s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, torch.tensor(P.vals, device="cuda")),
+                      order=order, precision=FP32_REFINE if args.pmc_probe == "fp32" else FP64)
+        s.factor(torch.tensor(P.diag, device="cuda"))
+        x = torch.tensor(P.rhs, device="cuda")
+        s.solve(x, refine=True)
+        torch.cuda.synchronize()
+        return
     P = W.poisson2d(N=args.mesh)
     precisions = {"auto": ["fp64", "fp32"], "both": ["fp64", "fp32"],
                   "fp64": ["fp64"], "fp32": ["fp32"]}[args.precision]
@@ -265,6 +346,15 @@ def main():
     head = "fp32" if "fp32" in results else "fp64"
     r = results[head]
     asm = bench_assembly(args, torch, cpu=not args.no_cpu_baseline)
+    rl = roofline(r, P)
+    if not args.no_traffic:
+        try:
+            t, detail = pmc_traffic(rl["kernel"], head)
+            rl["traffic"] = t
+            rl["traffic_over_algorithmic"] = t / rl["algorithmic_per_launch"]
+            rl["traffic_pmc"] = detail
+        except Exception as e:  # the counters are a diagnostic, not the measurement
+            print(f"PMC traffic unavailable: {e}", file=sys.stderr)
     perm = r["s"].get_perm()[0]
     cpu = None if args.no_cpu_baseline else cpu_baseline(P, perm, args.cpu_reps)
     out = {
@@ -297,7 +387,7 @@ def main():
             for p, v in results.items()
         },
         "kernels": {k: {"ms": round(t, 3), "launches": n} for k, (t, n) in r["kt"].items()},
-        "roofline": roofline(r, P),
+        "roofline": rl,
         "assembly": asm,
         "cpu_baseline": cpu,
     }
