@@ -1579,10 +1579,14 @@ __device__ inline int block_end(const int* ptype, int jb, int npiv, int chol) {
   return je;
 }
 
-template <class T>
-__global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
-                                                  double* x, int ldx, int nrhs,
-                                                  long long ustride, int chol) {
+// BIG nodes (the large fronts, few per level) solve only L11 here: the
+// rows below the pivots are updated by k_fwd_gemv over many workgroups, from
+// the pivot solution y stashed in Y and the gathered rows left in U.
+template <class T, int NTH, bool BIG>
+__global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
+                                                 double* x, int ldx, int nrhs,
+                                                 long long ustride, int chol, double* Y,
+                                                 long long ystride) {
   extern __shared__ double t[];
   const int p = nodes[blockIdx.x];
   const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
@@ -1602,14 +1606,14 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
   const int nch = c1 - c0;
   const bool cached = nch <= kChildCache;
   if (cached)
-    for (int q = tid; q < nch; q += kTPB) {
+    for (int q = tid; q < nch; q += NTH) {
       s_cv[q] = load_child<T>(P, P.ch[c0 + q]);
       s_uo[q] = P.uoff[P.ch[c0 + q]];
     }
   __syncthreads();
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
-    for (int i = tid; i < m; i += kTPB) {
+    for (int i = tid; i < m; i += NTH) {
       const int pi = i < fs ? pre[i] : i;
       double v = pi < ns ? xr[P.node_to_var[vp + pi]] : 0.0;
       for (int q = 0; q < nch; q++) {
@@ -1632,7 +1636,7 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
       const int je = block_end(ptype, jb, npiv, chol);
       const int bs = je - jb;
       double* Lb = t + P.lds_t;  // kSB x kSB block, column major
-      for (int e = tid; e < bs * bs; e += kTPB) {
+      for (int e = tid; e < bs * bs; e += NTH) {
         const int c = e / bs, rr = e - c * bs;
         Lb[rr + c * kSB] = (double)L[jb + rr + (size_t)(jb + c) * m];
       }
@@ -1657,18 +1661,18 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
         if (lane < bs) t[row] = tr;
       }
       __syncthreads();
-      for (int i = je + tid; i < m; i += kTPB) {
-        double s = 0.0;
+      for (int i = je + tid; i < (BIG ? npiv : m); i += NTH)
         t[i] = sub_dot(t[i], L + i + (size_t)jb * m, m, t + jb, je - jb);
-      }
       __syncthreads();
       jb = je;
     }
     double* U = P.U + P.uoff[p] + (long long)r * ustride;
-    for (int i = npiv + tid; i < m; i += kTPB) U[i - npiv] = t[i];
+    for (int i = npiv + tid; i < m; i += NTH) U[i - npiv] = t[i];
+    if (BIG)
+      for (int j = tid; j < npiv; j += NTH) Y[(long long)r * ystride + P.ioff[p] + j] = t[j];
     if (!chol) {
       // D solve, one thread per 1x1 pivot or 2x2 block
-      for (int j = tid; j < npiv; j += kTPB) {
+      for (int j = tid; j < npiv; j += NTH) {
         const int ty = ptype[j];
         if (ty == 1) {
           t[j] = t[j] / dv[j];
@@ -1682,7 +1686,7 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
       }
       __syncthreads();
     }
-    for (int j = tid; j < npiv; j += kTPB) xr[unpiv(vars[j])] = t[j];
+    for (int j = tid; j < npiv; j += NTH) xr[unpiv(vars[j])] = t[j];
     __syncthreads();
   }
 }
@@ -1690,10 +1694,12 @@ __global__ void __launch_bounds__(kTPB) k_forward(DevPlan P, const int* nodes,
 // Backward: t1 = x[pivots] - L21^T x[rows below], then the L11^T back solve
 // in blocks from the bottom (dot products over the rows below a block by
 // the whole workgroup, the diagonal block by one wave in registers).
-template <class T>
-__global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
-                                                   double* x, int ldx, int nrhs,
-                                                   int chol) {
+// BIG nodes take L21^T x from Y, computed by k_bwd_gemv over many
+// workgroups.
+template <class T, int NTH, bool BIG>
+__global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
+                                                  double* x, int ldx, int nrhs,
+                                                  int chol, const double* Y, long long ystride) {
   extern __shared__ double t[];
   const int p = nodes[blockIdx.x];
   const int tid = threadIdx.x;
@@ -1707,17 +1713,20 @@ __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
   const int w = tid / kWave, lane = tid % kWave;
   for (int r = 0; r < nrhs; r++) {
     double* xr = x + (size_t)r * ldx;
-    for (int i = tid; i < m; i += kTPB) {
+    for (int i = tid; i < m; i += NTH) {
       int v = i < npiv ? unpiv(vars[i]) : (i < fs ? vars[i] : P.contrib_rows[cp + i - fs]);
       t[i] = xr[v];
     }
     __syncthreads();
     // t[j] -= sum_{i >= npiv} L[i,j] t[i]  (one wave per column)
-    for (int j = w; j < npiv; j += kTPB / kWave) {
-      double s = 0.0;
-      s = strided_dot(L + (size_t)j * m, t, npiv + lane, m, kWave);
-      for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) t[j] -= s;
+    if (BIG) {
+      for (int j = tid; j < npiv; j += NTH) t[j] -= Y[(long long)r * ystride + P.ioff[p] + j];
+    } else {
+      for (int j = w; j < npiv; j += NTH / kWave) {
+        double s = strided_dot(L + (size_t)j * m, t, npiv + lane, m, kWave);
+        for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) t[j] -= s;
+      }
     }
     __syncthreads();
     for (int je = npiv; je > 0;) {
@@ -1726,16 +1735,15 @@ __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
       const int bs = je - jb;
       // rows of L11 below the block are final: dot products per column
       if (je < npiv) {
-        for (int j = jb + w; j < je; j += kTPB / kWave) {
-          double s = 0.0;
-          s = strided_dot(L + (size_t)j * m, t, je + lane, npiv, kWave);
+        for (int j = jb + w; j < je; j += NTH / kWave) {
+          double s = strided_dot(L + (size_t)j * m, t, je + lane, npiv, kWave);
           for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
           if (lane == 0) t[j] -= s;
         }
         __syncthreads();
       }
       double* Lb = t + P.lds_t;
-      for (int e = tid; e < bs * bs; e += kTPB) {
+      for (int e = tid; e < bs * bs; e += NTH) {
         const int c = e / bs, rr = e - c * bs;
         Lb[rr + c * kSB] = (double)L[jb + rr + (size_t)(jb + c) * m];
       }
@@ -1764,7 +1772,105 @@ __global__ void __launch_bounds__(kTPB) k_backward(DevPlan P, const int* nodes,
       __syncthreads();
       je = jb;
     }
-    for (int j = tid; j < npiv; j += kTPB) xr[unpiv(vars[j])] = t[j];
+    for (int j = tid; j < npiv; j += NTH) xr[unpiv(vars[j])] = t[j];
+    __syncthreads();
+  }
+}
+
+
+// Rows below the pivots of a BIG node, forward: U[i - npiv] -= sum_c
+// L[i, c] y[c] for 64 rows per workgroup (one row per lane), the pivot
+// columns split over the 4 waves and the partial sums added in wave order.
+constexpr int kGemvRows = 64;
+template <class T>
+__global__ void __launch_bounds__(kTPB) k_fwd_gemv(DevPlan P, const int* wnode,
+                                                   const int* wtile, int nrhs,
+                                                   long long ustride, const double* Y,
+                                                   long long ystride) {
+  extern __shared__ double ys[];  // y of the node, then partial sums [4][64]
+  const int p = wnode[blockIdx.x];
+  const int m = P.m[p], npiv = P.npiv[p];
+  const int i0 = npiv + kGemvRows * wtile[blockIdx.x];
+  if (i0 >= m) return;
+  const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+  const int i = i0 + lane;
+  const T* L = front_ptr<T>(P, p);
+  double* part = ys + ((npiv + 31) & ~31);
+  const int cw = (npiv + 3) / 4;
+  const int c0 = w * cw, c1 = min(npiv, c0 + cw);
+  for (int r = 0; r < nrhs; r++) {
+    const double* y = Y + (long long)r * ystride + P.ioff[p];
+    for (int c = tid; c < npiv; c += kTPB) ys[c] = y[c];
+    __syncthreads();
+    double acc = 0.0;
+    if (i < m) {
+      const T* Li = L + i;
+      int c = c0;
+      for (; c + 8 <= c1; c += 8) {
+        double l[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) l[u] = (double)Li[(size_t)(c + u) * m];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = __builtin_fma(l[u], ys[c + u], acc);
+      }
+      for (; c < c1; c++) acc = __builtin_fma((double)Li[(size_t)c * m], ys[c], acc);
+    }
+    part[w * kWave + lane] = acc;
+    __syncthreads();
+    if (w == 0 && i < m) {
+      const double s = ((part[lane] + part[kWave + lane]) + part[2 * kWave + lane]) +
+                       part[3 * kWave + lane];
+      P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)] -= s;
+    }
+    __syncthreads();
+  }
+}
+
+// Pivot columns of a BIG node, backward: Y[j] = sum_{i >= npiv} L[i, j] x[i]
+// for 16 columns per workgroup (4 per wave, lanes over the rows, coalesced),
+// with the node's x rows staged in LDS.
+constexpr int kGemvCols = 16;
+template <class T>
+__global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
+                                                   const int* wtile, const double* x,
+                                                   int ldx, int nrhs, double* Y,
+                                                   long long ystride) {
+  extern __shared__ double xs[];  // x of the rows below the pivots
+  const int p = wnode[blockIdx.x];
+  const int m = P.m[p], npiv = P.npiv[p];
+  const int j0 = kGemvCols * wtile[blockIdx.x];
+  if (j0 >= npiv) return;
+  const int fs = P.ns[p] + P.din[p];
+  const int cp = P.cptr[p];
+  const int* vars = P.vars + P.ioff[p];
+  const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+  const T* L = front_ptr<T>(P, p);
+  const int nr = m - npiv;
+  for (int r = 0; r < nrhs; r++) {
+    const double* xr = x + (size_t)r * ldx;
+    for (int q = tid; q < nr; q += kTPB) {
+      const int i = npiv + q;
+      xs[q] = xr[i < fs ? vars[i] : P.contrib_rows[cp + i - fs]];
+    }
+    __syncthreads();
+    for (int jj = 0; jj < kGemvCols / 4; jj++) {
+      const int j = j0 + 4 * w + jj;
+      if (j >= npiv) break;
+      const T* Lj = L + (size_t)j * m + npiv;
+      double s = 0.0;
+      int q = lane;
+      for (; q + 3 * kWave < nr; q += 4 * kWave) {
+        const double a0 = (double)Lj[q], a1 = (double)Lj[q + kWave];
+        const double a2 = (double)Lj[q + 2 * kWave], a3 = (double)Lj[q + 3 * kWave];
+        s = __builtin_fma(a0, xs[q], s);
+        s = __builtin_fma(a1, xs[q + kWave], s);
+        s = __builtin_fma(a2, xs[q + 2 * kWave], s);
+        s = __builtin_fma(a3, xs[q + 3 * kWave], s);
+      }
+      for (; q < nr; q += kWave) s = __builtin_fma((double)Lj[q], xs[q], s);
+      for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if (lane == 0) Y[(long long)r * ystride + P.ioff[p] + j] = s;
+    }
     __syncthreads();
   }
 }
@@ -1891,6 +1997,13 @@ struct Handle {
   std::vector<int> upd_ptr, upd_node, upd_tile;  // upd_ptr per (level, panel)
   std::vector<int> upd_lvl;                      // first upd_ptr slot of a level
   DBuf<int> d_small_nodes, d_large_nodes, d_upd_node, d_upd_tile;
+  // solve: per level, the nodes solved by one workgroup each (sv) and the
+  // GEMV work lists of the large nodes (fg: 64-row tiles forward, bg:
+  // 16-column tiles backward); Y holds a large node's pivot vector
+  std::vector<int> sv_ptr, sv_nodes, fg_ptr, fg_node, fg_tile, bg_ptr, bg_node, bg_tile;
+  DBuf<int> d_sv_nodes, d_fg_node, d_fg_tile, d_bg_node, d_bg_tile;
+  DBuf<double> d_Y;
+  int y_nrhs = 0;
   DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_norm, d_dv, d_dv2;
   int esize = 8;  // bytes per front entry: 8 (FP64) or 4 (FP32 factor)
   DBuf<int> d_vars, d_pre, d_ptype;
@@ -2031,6 +2144,33 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     }
   }
   H.upd_lvl[nlev] = (int)H.upd_ptr.size() - 1;
+  H.sv_ptr.assign(nlev + 1, 0);
+  H.fg_ptr.assign(nlev + 1, 0);
+  H.bg_ptr.assign(nlev + 1, 0);
+  H.sv_nodes.clear();
+  H.fg_node.clear();
+  H.fg_tile.clear();
+  H.bg_node.clear();
+  H.bg_tile.clear();
+  for (int l = 0; l < nlev; l++) {
+    for (int q = H.lds_ptr[l]; q < H.lds_ptr[l + 1]; q++) H.sv_nodes.push_back(H.lds_nodes[q]);
+    for (int q = H.small_ptr[l]; q < H.small_ptr[l + 1]; q++)
+      H.sv_nodes.push_back(H.small_nodes[q]);
+    for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
+      const int k = H.large_nodes[q];
+      for (int t = 0; t < (H.cap[k] + kGemvRows - 1) / kGemvRows; t++) {
+        H.fg_node.push_back(k);
+        H.fg_tile.push_back(t);
+      }
+      for (int t = 0; t < (H.cap[k] + kGemvCols - 1) / kGemvCols; t++) {
+        H.bg_node.push_back(k);
+        H.bg_tile.push_back(t);
+      }
+    }
+    H.sv_ptr[l + 1] = (int)H.sv_nodes.size();
+    H.fg_ptr[l + 1] = (int)H.fg_node.size();
+    H.bg_ptr[l + 1] = (int)H.bg_node.size();
+  }
 }
 
 // Static maps that do not depend on the slack.
@@ -2137,6 +2277,12 @@ static void upload_static(Handle& H) {
   H.d_large_nodes.upload(H.large_nodes);
   H.d_upd_node.upload(H.upd_node);
   H.d_upd_tile.upload(H.upd_tile);
+  H.d_sv_nodes.upload(H.sv_nodes);
+  H.d_fg_node.upload(H.fg_node);
+  H.d_fg_tile.upload(H.fg_tile);
+  H.d_bg_node.upload(H.bg_node);
+  H.d_bg_tile.upload(H.bg_tile);
+  H.y_nrhs = 0;
   H.d_F.alloc(((size_t)H.f_total * H.esize + 7) / 8);
   H.d_dv.alloc((size_t)H.i_total);
   H.d_dv2.alloc((size_t)H.i_total);
@@ -2452,6 +2598,10 @@ static void ensure_u(Handle& H, int nrhs) {
 template <class T>
 static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStream_t st) {
   ensure_u(H, nrhs);
+  if (H.y_nrhs < nrhs) {
+    H.d_Y.alloc((size_t)H.i_total * nrhs);
+    H.y_nrhs = nrhs;
+  }
   DevPlan P = dev_plan(H);
   const int nlev = (int)H.lvl_ptr.size() - 1;
   const int chol = H.opt.solver_type == KKT_LDL_CHOLESKY;
@@ -2459,17 +2609,40 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   P.lds_t = lds_t;
   size_t lds = sizeof(double) * ((size_t)lds_t + (size_t)kSB * kSB);
   if (lds > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
+  const size_t lds_fg = sizeof(double) * ((size_t)lds_t + 4 * kWave);
+  const size_t lds_bg = sizeof(double) * (size_t)H.max_cap;
+  const long long ys = H.i_total, us = H.u_total;
+  double* Y = H.d_Y.p;
+  constexpr int kBigThreads = 1024;
   for (int l = 0; l < nlev; l++) {
-    int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
     ProfScope ps_(H, st, KC_FORWARD);
-    hipLaunchKernelGGL(k_forward<T>, dim3(n1 - n0), dim3(kTPB), lds, st, P,
-                       H.d_lvl_nodes.p + n0, dx, ldx, nrhs, (long long)H.u_total, chol);
+    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
+    if (s1 > s0)
+      hipLaunchKernelGGL((k_forward<T, kTPB, false>), dim3(s1 - s0), dim3(kTPB), lds, st, P,
+                         H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol, Y, ys);
+    const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
+    if (g1 > g0) {
+      hipLaunchKernelGGL((k_forward<T, kBigThreads, true>), dim3(g1 - g0), dim3(kBigThreads), lds,
+                         st, P, H.d_large_nodes.p + g0, dx, ldx, nrhs, us, chol, Y, ys);
+      const int f0 = H.fg_ptr[l], f1 = H.fg_ptr[l + 1];
+      hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(f1 - f0), dim3(kTPB), lds_fg, st, P,
+                         H.d_fg_node.p + f0, H.d_fg_tile.p + f0, nrhs, us, Y, ys);
+    }
   }
   for (int l = nlev - 1; l >= 0; l--) {
-    int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
     ProfScope ps_(H, st, KC_BACKWARD);
-    hipLaunchKernelGGL(k_backward<T>, dim3(n1 - n0), dim3(kTPB), lds, st, P,
-                       H.d_lvl_nodes.p + n0, dx, ldx, nrhs, chol);
+    const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
+    if (g1 > g0) {
+      const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
+      hipLaunchKernelGGL(k_bwd_gemv<T>, dim3(b1 - b0), dim3(kTPB), lds_bg, st, P,
+                         H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys);
+      hipLaunchKernelGGL((k_backward<T, kBigThreads, true>), dim3(g1 - g0), dim3(kBigThreads),
+                         lds, st, P, H.d_large_nodes.p + g0, dx, ldx, nrhs, chol, Y, ys);
+    }
+    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
+    if (s1 > s0)
+      hipLaunchKernelGGL((k_backward<T, kTPB, false>), dim3(s1 - s0), dim3(kTPB), lds, st, P,
+                         H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol, Y, ys);
   }
   HIPCHECK(hipGetLastError());
 }
