@@ -32,7 +32,9 @@ enum { KKT_LDL_LDL = 0, KKT_LDL_CHOLESKY = 1 };
 enum { KKT_LDL_FP64 = 0, KKT_LDL_FP32_REFINE = 1 };
 /* Memory space of caller buffers. */
 enum { KKT_LDL_HOST = 0, KKT_LDL_DEVICE = 1 };
-/* Positive status codes (warnings). */
+/* Positive status codes (warnings).  KKT_LDL_WARN_SINGULAR is reserved: a
+ * singular matrix factors without error, as in the reference (zero pivots
+ * count positive in the inertia). */
 enum { KKT_LDL_WARN_SINGULAR = 1, KKT_LDL_WARN_NOT_CONVERGED = 2 };
 
 typedef struct {
