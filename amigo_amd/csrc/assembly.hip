@@ -115,10 +115,8 @@ int kkt_asm_create(int n, const int* rowp, const int* cols, long long ncontrib,
     for (long long q = 0; q < ncontrib; q++)
       if (slot[q] >= 0) order[fill[slot[q]]++] = (int)q;
     std::vector<unsigned char> isfix(n, 0), mode(std::max(1, h.nnz), kktasm::kSum);
-    for (int i = 0; i < nfixed; i++) {
-      if (fixed[i] < 0 || fixed[i] >= n) throw std::runtime_error("fixed index out of range");
-      isfix[fixed[i]] = 1;
-    }
+    for (int i = 0; i < nfixed; i++)  // out-of-range indices are ignored, as the reference does
+      if (fixed[i] >= 0 && fixed[i] < n) isfix[fixed[i]] = 1;
     for (int r = 0; r < n; r++)
       for (int p = rowp[r]; p < rowp[r + 1]; p++)
         if (isfix[r] || isfix[cols[p]])

@@ -122,6 +122,10 @@ struct DevPlan {
   double* dv2;         // FP64 off-diagonal d21 of a 2x2 block, at its first position
   double* W;
   double* U;
+  // large nodes: inverses of the kPB x kPB diagonal blocks of L11, two
+  // copies per block (column major, then transposed), at pinvoff[node]
+  double* pinv;
+  const long long* pinvoff;
   // numeric input
   const double* vals;
   const double* diag;
@@ -1579,9 +1583,9 @@ __device__ inline int block_end(const int* ptype, int jb, int npiv, int chol) {
   return je;
 }
 
-// BIG nodes (the large fronts, few per level) solve only L11 here: the
-// rows below the pivots are updated by k_fwd_gemv over many workgroups, from
-// the pivot solution y stashed in Y and the gathered rows left in U.
+// BIG nodes (the large fronts, few per level) only gather here: the
+// pivot rows of the front vector go to Y and the rows below to U, and the
+// panel steps (k_fwd_step) solve them over many workgroups.
 template <class T, int NTH, bool BIG>
 __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
                                                  double* x, int ldx, int nrhs,
@@ -1632,6 +1636,16 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
       t[i] = v;
     }
     __syncthreads();
+    if (BIG) {
+      double* U = P.U + P.uoff[p] + (long long)r * ustride;
+      double* y = Y + (long long)r * ystride + P.ioff[p];
+      for (int i = tid; i < m; i += NTH) {
+        if (i < npiv) y[i] = t[i];
+        else U[i - npiv] = t[i];
+      }
+      __syncthreads();
+      continue;
+    }
     for (int jb = 0; jb < npiv;) {
       const int je = block_end(ptype, jb, npiv, chol);
       const int bs = je - jb;
@@ -1646,14 +1660,15 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
         double tr = lane < bs ? t[row] : 0.0;
         const double dr = (chol && lane < bs) ? dv[row] : 1.0;
         const int pt = (!chol && lane < bs) ? ptype[row] : 1;
+        // c is uniform: the broadcasts are readlanes, not LDS permutes
         for (int c = 0; c < bs; c++) {
-          const int ty = __shfl(pt, c);
+          const int ty = __builtin_amdgcn_readlane(pt, c);
           if (ty == 1) {
             if (chol && lane == c) tr = tr / dr;
-            const double tc = __shfl(tr, c);
+            const double tc = lane_d(tr, c);
             if (lane > c && lane < bs) tr -= Lb[lane + c * kSB] * tc;
           } else if (ty == 2) {
-            const double ta = __shfl(tr, c), tb = __shfl(tr, c + 1);
+            const double ta = lane_d(tr, c), tb = lane_d(tr, c + 1);
             if (lane > c + 1 && lane < bs)
               tr -= Lb[lane + c * kSB] * ta + Lb[lane + (c + 1) * kSB] * tb;
           }
@@ -1661,15 +1676,13 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
         if (lane < bs) t[row] = tr;
       }
       __syncthreads();
-      for (int i = je + tid; i < (BIG ? npiv : m); i += NTH)
+      for (int i = je + tid; i < m; i += NTH)
         t[i] = sub_dot(t[i], L + i + (size_t)jb * m, m, t + jb, je - jb);
       __syncthreads();
       jb = je;
     }
     double* U = P.U + P.uoff[p] + (long long)r * ustride;
     for (int i = npiv + tid; i < m; i += NTH) U[i - npiv] = t[i];
-    if (BIG)
-      for (int j = tid; j < npiv; j += NTH) Y[(long long)r * ystride + P.ioff[p] + j] = t[j];
     if (!chol) {
       // D solve, one thread per 1x1 pivot or 2x2 block
       for (int j = tid; j < npiv; j += NTH) {
@@ -1694,12 +1707,10 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
 // Backward: t1 = x[pivots] - L21^T x[rows below], then the L11^T back solve
 // in blocks from the bottom (dot products over the rows below a block by
 // the whole workgroup, the diagonal block by one wave in registers).
-// BIG nodes take L21^T x from Y, computed by k_bwd_gemv over many
-// workgroups.
-template <class T, int NTH, bool BIG>
+// BIG nodes are solved by k_bwd_gemv and the panel steps (k_bwd_step).
+template <class T, int NTH>
 __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
-                                                  double* x, int ldx, int nrhs,
-                                                  int chol, const double* Y, long long ystride) {
+                                                  double* x, int ldx, int nrhs, int chol) {
   extern __shared__ double t[];
   const int p = nodes[blockIdx.x];
   const int tid = threadIdx.x;
@@ -1719,14 +1730,10 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
     }
     __syncthreads();
     // t[j] -= sum_{i >= npiv} L[i,j] t[i]  (one wave per column)
-    if (BIG) {
-      for (int j = tid; j < npiv; j += NTH) t[j] -= Y[(long long)r * ystride + P.ioff[p] + j];
-    } else {
-      for (int j = w; j < npiv; j += NTH / kWave) {
-        double s = strided_dot(L + (size_t)j * m, t, npiv + lane, m, kWave);
-        for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) t[j] -= s;
-      }
+    for (int j = w; j < npiv; j += NTH / kWave) {
+      double s = strided_dot(L + (size_t)j * m, t, npiv + lane, m, kWave);
+      for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if (lane == 0) t[j] -= s;
     }
     __syncthreads();
     for (int je = npiv; je > 0;) {
@@ -1756,14 +1763,14 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
         double acc = 0.0;
         const int pt = (!chol && lane < bs) ? ptype[col] : 1;
         for (int i = bs - 1; i >= 0; i--) {
-          const int ty = __shfl(pt, i);
+          const int ty = __builtin_amdgcn_readlane(pt, i);
           if (ty == 1) {
             if (lane == i) tv = chol ? (tv - acc) / dc : tv - acc;
-            const double ti = __shfl(tv, i);
+            const double ti = lane_d(tv, i);
             if (lane < i) acc += Lb[i + lane * kSB] * ti;
           } else if (ty == 3) {
             if (lane == i || lane == i - 1) tv -= acc;
-            const double ti = __shfl(tv, i), tj = __shfl(tv, i - 1);
+            const double ti = lane_d(tv, i), tj = lane_d(tv, i - 1);
             if (lane < i - 1) acc += Lb[i + lane * kSB] * ti + Lb[i - 1 + lane * kSB] * tj;
           }
         }
@@ -1778,57 +1785,244 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
 }
 
 
-// Rows below the pivots of a BIG node, forward: U[i - npiv] -= sum_c
-// L[i, c] y[c] for 64 rows per workgroup (one row per lane), the pivot
-// columns split over the 4 waves and the partial sums added in wave order.
-constexpr int kGemvRows = 64;
+// --------------------------------------------------------------------------
+// Large-node solves by partitioned inverse.  The pivot columns of a large
+// node are cut into panels of kPB columns.  After the factorization the
+// diagonal block B_s of every panel is inverted once by k_panel_inv.  B_s is
+// unit lower triangular, with the slot L[j + 1, j] of every 2x2 pivot (j,
+// j + 1) taken as zero: that slot does not belong to L (D holds the
+// coupling), also where the pair is split by a panel boundary.  A solve then walks the panels with one launch per panel
+// for all large nodes of a level: every workgroup forms x_s = B_s^{-1} y_s
+// (forward) or B_s^{-T} y_s (backward) itself, from the kPB entries of y_s
+// and the kPB x kPB inverse, and applies x_s to its own tile of the rows
+// below the panel (forward) or of the columns before it (backward).  A
+// panel step therefore costs one memory round trip spread over many CUs,
+// instead of kPB dependent column steps inside one workgroup.
+// --------------------------------------------------------------------------
+constexpr int kPB = 64;        // panel width of the partitioned inverse
+constexpr int kStepRows = 64;  // forward step: rows per workgroup (one per lane)
+constexpr int kStepCols = 32;  // backward step: columns per workgroup (8 per wave)
+constexpr long long kPinvPanel = 2LL * kPB * kPB;  // doubles per panel: B^{-1}, B^{-T}
+
+// One wave per (node, panel s): lane c computes column c of B_s^{-1} by
+// forward substitution against B_s held in LDS; the inverse is written
+// column major (lanes = rows) and transposed (lanes = columns), so that both
+// solve directions read it coalesced.  Nodes that failed (stat != 0) are
+// refactored on a new plan and skipped here.
 template <class T>
-__global__ void __launch_bounds__(kTPB) k_fwd_gemv(DevPlan P, const int* wnode,
-                                                   const int* wtile, int nrhs,
-                                                   long long ustride, const double* Y,
-                                                   long long ystride) {
-  extern __shared__ double ys[];  // y of the node, then partial sums [4][64]
-  const int p = wnode[blockIdx.x];
+__global__ void __launch_bounds__(kWave) k_panel_inv(DevPlan P, const int* inode,
+                                                     const int* ipanel) {
+  __shared__ double Bs[kPB * kPB];        // B[i][j] at i + j * kPB
+  __shared__ double Zs[kPB * (kPB + 1)];  // B^{-1}[i][c] at i * (kPB + 1) + c
+  const int p = inode[blockIdx.x], s = ipanel[blockIdx.x];
+  if (P.stat[p] != 0) return;
   const int m = P.m[p], npiv = P.npiv[p];
-  const int i0 = npiv + kGemvRows * wtile[blockIdx.x];
-  if (i0 >= m) return;
-  const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
-  const int i = i0 + lane;
+  if (m > P.cap[p] || npiv > m) return;
+  const int b0 = s * kPB;
+  if (b0 >= npiv) return;
+  const int w = min(kPB, npiv - b0);
   const T* L = front_ptr<T>(P, p);
-  double* part = ys + ((npiv + 31) & ~31);
-  const int cw = (npiv + 3) / 4;
-  const int c0 = w * cw, c1 = min(npiv, c0 + cw);
-  for (int r = 0; r < nrhs; r++) {
-    const double* y = Y + (long long)r * ystride + P.ioff[p];
-    for (int c = tid; c < npiv; c += kTPB) ys[c] = y[c];
-    __syncthreads();
-    double acc = 0.0;
-    if (i < m) {
-      const T* Li = L + i;
-      int c = c0;
-      for (; c + 8 <= c1; c += 8) {
-        double l[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) l[u] = (double)Li[(size_t)(c + u) * m];
-#pragma unroll
-        for (int u = 0; u < 8; u++) acc = __builtin_fma(l[u], ys[c + u], acc);
-      }
-      for (; c < c1; c++) acc = __builtin_fma((double)Li[(size_t)c * m], ys[c], acc);
+  const int* ptype = P.ptype + P.ioff[p];
+  const int lane = threadIdx.x;
+  for (int j = 0; j < kPB; j++) {
+    const int i = lane;
+    double v = 0.0;
+    if (i < w && j < w) {
+      if (i == j) v = 1.0;
+      else if (i > j && !(i == j + 1 && ptype[b0 + j] == 2))
+        v = (double)L[(size_t)(b0 + i) + (size_t)(b0 + j) * m];
     }
-    part[w * kWave + lane] = acc;
+    Bs[i + j * kPB] = v;
+    Zs[j * (kPB + 1) + lane] = j == lane ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  // z_i = -sum_{j < i} B[i][j] z_j for i > c (z_c = 1, z_j = 0 for j < c);
+  // each lane touches only its own column of Z
+  for (int i = 1; i < w; i++) {
+    double acc = 0.0;
+    for (int j = 0; j < i; j++) acc += Bs[i + j * kPB] * Zs[j * (kPB + 1) + lane];
+    if (i > lane) Zs[i * (kPB + 1) + lane] = -acc;
+  }
+  __syncthreads();
+  double* inv = P.pinv + P.pinvoff[p] + (long long)s * kPinvPanel;
+  for (int c = 0; c < kPB; c++) inv[lane + c * kPB] = Zs[lane * (kPB + 1) + c];
+  for (int i = 0; i < kPB; i++) inv[kPB * kPB + lane + i * kPB] = Zs[i * (kPB + 1) + lane];
+}
+
+// Forward panel step s of the large nodes of a level.  Y holds the pivot
+// rows of each front vector (gathered by k_forward and updated by the
+// earlier steps), U the rows below the pivots.  Workgroup (node, tile):
+//   x_s = B_s^{-1} y[b0, b1)                         (lanes = rows of x_s)
+//   y/U[i] -= L[i, b0:b1] x_s  for its kStepRows rows i >= b1
+// Tile 0 also applies D^{-1} to x_s and writes the panel's entries of x.
+// A 2x2 pivot split by the panel boundary is finished by the next step,
+// which recomputes the first pivot's entry from the previous panel.
+template <class T>
+__global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
+                                                   const int* wtile, int s, double* x,
+                                                   int ldx, int nrhs, long long ustride,
+                                                   double* Y, long long ystride) {
+  constexpr int NW = kTPB / kWave;
+  constexpr int CQ = kPB / NW;  // panel columns per wave
+  __shared__ double xp[kPB];
+  __shared__ double part[NW][kWave];
+  const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
+  const int m = P.m[p], npiv = P.npiv[p];
+  const int b0 = s * kPB;
+  if (b0 >= npiv) return;
+  const int b1 = min(npiv, b0 + kPB), w = b1 - b0;
+  const int i0 = b1 + tile * kStepRows;
+  if (tile > 0 && i0 >= m) return;
+  const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
+  const T* L = front_ptr<T>(P, p);
+  const double* inv = P.pinv + P.pinvoff[p] + (long long)s * kPinvPanel;
+  const int i = i0 + lane;
+  const int* ptype = P.ptype + P.ioff[p];
+  // a 2x2 pivot split by the boundary: its coupling slot L[b1, b1 - 1] is
+  // not part of L and must not enter the update of row b1
+  const bool cut = b1 < npiv && ptype[b1 - 1] == 2;
+  // right-hand-side independent operands: L[i, panel] and B^{-1}[lane, panel],
+  // loaded once from clamped (always valid) addresses
+  const size_t irow = (size_t)min(i, m - 1);
+  double lrow[CQ], ainv[CQ];
+#pragma unroll
+  for (int q = 0; q < CQ; q++) {
+    const int c = wv * CQ + q;
+    const double lv = (double)L[irow + (size_t)(b0 + min(c, w - 1)) * m];
+    lrow[q] = (i < m && c < w && !(cut && i == b1 && c == w - 1)) ? lv : 0.0;
+    ainv[q] = c < w ? inv[lane + c * kPB] : 0.0;
+  }
+  const int* vars = P.vars + P.ioff[p];
+  const double* dv = P.dv + P.ioff[p];
+  const double* dv2 = P.dv2 + P.ioff[p];
+  for (int r = 0; r < nrhs; r++) {
+    double* y = Y + (long long)r * ystride + P.ioff[p];
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < CQ; q++) {
+      const int c = wv * CQ + q;
+      if (c < w) a += ainv[q] * y[b0 + c];
+    }
+    part[wv][lane] = a;
     __syncthreads();
-    if (w == 0 && i < m) {
-      const double s = ((part[lane] + part[kWave + lane]) + part[2 * kWave + lane]) +
-                       part[3 * kWave + lane];
-      P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)] -= s;
+    if (wv == 0)
+      xp[lane] = lane < w ? ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]
+                          : 0.0;
+    __syncthreads();
+    double g = 0.0;
+#pragma unroll
+    for (int q = 0; q < CQ; q++) g += lrow[q] * xp[wv * CQ + q];
+    part[wv][lane] = g;
+    __syncthreads();
+    if (wv == 0 && i < m) {
+      const double sum = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+      if (i < npiv) y[i] -= sum;
+      else P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)] -= sum;
+    }
+    if (tile == 0 && wv == 1 && lane < w) {
+      double* xr = x + (size_t)r * ldx;
+      const int j = b0 + lane;
+      const int ty = ptype[j];
+      if (ty == 1) {
+        xr[unpiv(vars[j])] = xp[lane] / dv[j];
+      } else if (ty == 2 && lane + 1 < w) {
+        const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
+        const double dinv = 1.0 / (d11 * d22 - d21 * d21);
+        const double ta = xp[lane], tb = xp[lane + 1];
+        xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
+        xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
+      } else if (ty == 3 && lane == 0 && s > 0) {
+        // first pivot of the pair is the last row of panel s-1
+        const double* invp = inv - kPinvPanel;
+        const double* yp = y + (b0 - kPB);
+        double ta = 0.0;
+        for (int c = 0; c < kPB; c++) ta += invp[(kPB - 1) + c * kPB] * yp[c];
+        const double d11 = dv[j - 1], d21 = dv2[j - 1], d22 = dv[j];
+        const double dinv = 1.0 / (d11 * d22 - d21 * d21);
+        const double tb = xp[0];
+        xr[unpiv(vars[j - 1])] = dinv * (d22 * ta - d21 * tb);
+        xr[unpiv(vars[j])] = dinv * (d11 * tb - d21 * ta);
+      }
     }
     __syncthreads();
   }
 }
 
-// Pivot columns of a BIG node, backward: Y[j] = sum_{i >= npiv} L[i, j] x[i]
-// for 16 columns per workgroup (4 per wave, lanes over the rows, coalesced),
-// with the node's x rows staged in LDS.
+// Backward panel step s: panel q = (number of panels) - 1 - s, from the
+// bottom.  Y holds the right-hand side of L11^T (k_bwd_gemv), updated by the
+// earlier steps.  Workgroup (node, tile):
+//   x_q = B_q^{-T} y[b0, b1)                         (lanes = rows of x_q)
+//   y[c] -= L[b0:b1, c]^T x_q  for its kStepCols columns c < b0
+// (one wave per column, lanes over the panel rows).  Tile 0 writes x_q.
+template <class T>
+__global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
+                                                   const int* wtile, int s, double* x,
+                                                   int ldx, int nrhs, double* Y,
+                                                   long long ystride) {
+  constexpr int NW = kTPB / kWave;
+  constexpr int CQ = kPB / NW;           // panel rows per wave in x_q = B^{-T} y
+  constexpr int CPW = kStepCols / NW;    // tile columns per wave
+  __shared__ double xp[kPB];
+  __shared__ double part[NW][kWave];
+  const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
+  const int m = P.m[p], npiv = P.npiv[p];
+  const int q = (npiv + kPB - 1) / kPB - 1 - s;
+  if (q < 0) return;
+  const int b0 = q * kPB, w = min(kPB, npiv - b0);
+  const int c0 = tile * kStepCols;
+  if (tile > 0 && c0 >= b0) return;
+  const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
+  const T* L = front_ptr<T>(P, p);
+  const double* invt = P.pinv + P.pinvoff[p] + (long long)q * kPinvPanel + kPB * kPB;
+  const size_t prow = (size_t)(b0 + min(lane, w - 1));
+  const int cmax = max(b0 - 1, 0);
+  // a 2x2 pivot split by the boundary: L[b0, b0 - 1] is not part of L
+  const bool cut = b0 > 0 && P.ptype[P.ioff[p] + b0 - 1] == 2;
+  double lcol[CPW], ainv[CQ];
+#pragma unroll
+  for (int k = 0; k < CPW; k++) {
+    const int c = c0 + wv * CPW + k;
+    const double lv = (double)L[prow + (size_t)min(c, cmax) * m];
+    lcol[k] = (lane < w && c < b0 && !(cut && lane == 0 && c == b0 - 1)) ? lv : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < CQ; k++) {
+    const int i = wv * CQ + k;
+    ainv[k] = i < w ? invt[lane + i * kPB] : 0.0;
+  }
+  const int* vars = P.vars + P.ioff[p];
+  for (int r = 0; r < nrhs; r++) {
+    double* y = Y + (long long)r * ystride + P.ioff[p];
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < CQ; k++) {
+      const int i = wv * CQ + k;
+      if (i < w) a += ainv[k] * y[b0 + i];
+    }
+    part[wv][lane] = a;
+    __syncthreads();
+    if (wv == 0)
+      xp[lane] = lane < w ? ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]
+                          : 0.0;
+    __syncthreads();
+    const double xl = xp[lane];
+#pragma unroll
+    for (int k = 0; k < CPW; k++) {
+      double g = lcol[k] * xl;
+      for (int off = kWave / 2; off > 0; off >>= 1) g += __shfl_xor(g, off);
+      const int c = c0 + wv * CPW + k;
+      if (lane == 0 && c < b0) y[c] -= g;
+    }
+    if (tile == 0 && wv == 1 && lane < w)
+      x[(size_t)r * ldx + unpiv(vars[b0 + lane])] = xp[lane];
+    __syncthreads();
+  }
+}
+
+// Pivot columns of a BIG node, backward: Y[j] = x[pivot j] - sum_{i >= npiv}
+// L[i, j] x[i], the right-hand side of L11^T, for 16 columns per workgroup
+// (4 per wave, lanes over the rows, coalesced), with the node's x rows
+// staged in LDS.
 constexpr int kGemvCols = 16;
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
@@ -1869,7 +2063,7 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
       }
       for (; q < nr; q += kWave) s = __builtin_fma((double)Lj[q], xs[q], s);
       for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) Y[(long long)r * ystride + P.ioff[p] + j] = s;
+      if (lane == 0) Y[(long long)r * ystride + P.ioff[p] + j] = xr[unpiv(vars[j])] - s;
     }
     __syncthreads();
   }
@@ -1959,7 +2153,7 @@ struct DBuf {
 
 enum KernelClass {
   KC_PREPARE, KC_ASSEMBLE, KC_FRONT_LDS, KC_FACTOR, KC_PANEL, KC_UPDATE,
-  KC_FORWARD, KC_BACKWARD, KC_REFINE, KC_COUNT
+  KC_FORWARD, KC_BACKWARD, KC_REFINE, KC_INVERT, KC_COUNT
 };
 
 struct Handle {
@@ -1997,12 +2191,22 @@ struct Handle {
   std::vector<int> upd_ptr, upd_node, upd_tile;  // upd_ptr per (level, panel)
   std::vector<int> upd_lvl;                      // first upd_ptr slot of a level
   DBuf<int> d_small_nodes, d_large_nodes, d_upd_node, d_upd_tile;
-  // solve: per level, the nodes solved by one workgroup each (sv) and the
-  // GEMV work lists of the large nodes (fg: 64-row tiles forward, bg:
-  // 16-column tiles backward); Y holds a large node's pivot vector
-  std::vector<int> sv_ptr, sv_nodes, fg_ptr, fg_node, fg_tile, bg_ptr, bg_node, bg_tile;
-  DBuf<int> d_sv_nodes, d_fg_node, d_fg_tile, d_bg_node, d_bg_tile;
+  // solve: per level, the nodes solved by one workgroup each (sv), the
+  // backward L21^T GEMV tiles of the large nodes (bg, 16 columns each) and
+  // their panel steps (fs forward, bs backward: fs_lvl[l] is the first step
+  // slot of level l, a slot's work items are [fs_ptr[slot], fs_ptr[slot+1]));
+  // Y holds a large node's pivot vector
+  std::vector<int> sv_ptr, sv_nodes, bg_ptr, bg_node, bg_tile;
+  std::vector<int> fs_lvl, fs_ptr, fs_node, fs_tile, bs_lvl, bs_ptr, bs_node, bs_tile;
+  DBuf<int> d_sv_nodes, d_bg_node, d_bg_tile, d_fs_node, d_fs_tile, d_bs_node, d_bs_tile;
   DBuf<double> d_Y;
+  // panel inverses of the large nodes: one (node, panel) item per block
+  std::vector<long long> pinvoff;
+  std::vector<int> inv_node, inv_panel;
+  long long pinv_total = 0;
+  DBuf<long long> d_pinvoff;
+  DBuf<int> d_inv_node, d_inv_panel;
+  DBuf<double> d_pinv;
   int y_nrhs = 0;
   DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_norm, d_dv, d_dv2;
   int esize = 8;  // bytes per front entry: 8 (FP64) or 4 (FP32 factor)
@@ -2145,32 +2349,78 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   }
   H.upd_lvl[nlev] = (int)H.upd_ptr.size() - 1;
   H.sv_ptr.assign(nlev + 1, 0);
-  H.fg_ptr.assign(nlev + 1, 0);
   H.bg_ptr.assign(nlev + 1, 0);
   H.sv_nodes.clear();
-  H.fg_node.clear();
-  H.fg_tile.clear();
   H.bg_node.clear();
   H.bg_tile.clear();
+  H.fs_lvl.assign(nlev + 1, 0);
+  H.bs_lvl.assign(nlev + 1, 0);
+  H.fs_ptr.assign(1, 0);
+  H.bs_ptr.assign(1, 0);
+  H.fs_node.clear();
+  H.fs_tile.clear();
+  H.bs_node.clear();
+  H.bs_tile.clear();
+  H.pinvoff.assign(N, 0);
+  H.inv_node.clear();
+  H.inv_panel.clear();
+  long long pv = 0;
+  // panels of a large node: its pivots never exceed cap - cs
+  auto npanels = [&](int k) { return (H.cap[k] - H.cs[k] + kPB - 1) / kPB; };
   for (int l = 0; l < nlev; l++) {
     for (int q = H.lds_ptr[l]; q < H.lds_ptr[l + 1]; q++) H.sv_nodes.push_back(H.lds_nodes[q]);
     for (int q = H.small_ptr[l]; q < H.small_ptr[l + 1]; q++)
       H.sv_nodes.push_back(H.small_nodes[q]);
+    int maxp = 0;
     for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
       const int k = H.large_nodes[q];
-      for (int t = 0; t < (H.cap[k] + kGemvRows - 1) / kGemvRows; t++) {
-        H.fg_node.push_back(k);
-        H.fg_tile.push_back(t);
-      }
       for (int t = 0; t < (H.cap[k] + kGemvCols - 1) / kGemvCols; t++) {
         H.bg_node.push_back(k);
         H.bg_tile.push_back(t);
       }
+      H.pinvoff[k] = pv;
+      pv += npanels(k) * kPinvPanel;
+      for (int t = 0; t < npanels(k); t++) {
+        H.inv_node.push_back(k);
+        H.inv_panel.push_back(t);
+      }
+      maxp = std::max(maxp, npanels(k));
     }
     H.sv_ptr[l + 1] = (int)H.sv_nodes.size();
-    H.fg_ptr[l + 1] = (int)H.fg_node.size();
     H.bg_ptr[l + 1] = (int)H.bg_node.size();
+    // forward step t: rows [b1, m) with b1 > t*kPB; tile 0 always exists
+    H.fs_lvl[l] = (int)H.fs_ptr.size() - 1;
+    for (int t = 0; t < maxp; t++) {
+      for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
+        const int k = H.large_nodes[q];
+        if (t >= npanels(k)) continue;
+        const int nt = std::max(1, (H.cap[k] - t * kPB + kStepRows - 1) / kStepRows);
+        for (int u = 0; u < nt; u++) {
+          H.fs_node.push_back(k);
+          H.fs_tile.push_back(u);
+        }
+      }
+      H.fs_ptr.push_back((int)H.fs_node.size());
+    }
+    // backward step t: panel (panels - 1 - t), columns before it
+    H.bs_lvl[l] = (int)H.bs_ptr.size() - 1;
+    for (int t = 0; t < maxp; t++) {
+      for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
+        const int k = H.large_nodes[q];
+        if (t >= npanels(k)) continue;
+        const int cols = (npanels(k) - 1 - t) * kPB;
+        const int nt = std::max(1, (cols + kStepCols - 1) / kStepCols);
+        for (int u = 0; u < nt; u++) {
+          H.bs_node.push_back(k);
+          H.bs_tile.push_back(u);
+        }
+      }
+      H.bs_ptr.push_back((int)H.bs_node.size());
+    }
   }
+  H.fs_lvl[nlev] = (int)H.fs_ptr.size() - 1;
+  H.bs_lvl[nlev] = (int)H.bs_ptr.size() - 1;
+  H.pinv_total = pv;
 }
 
 // Static maps that do not depend on the slack.
@@ -2278,10 +2528,16 @@ static void upload_static(Handle& H) {
   H.d_upd_node.upload(H.upd_node);
   H.d_upd_tile.upload(H.upd_tile);
   H.d_sv_nodes.upload(H.sv_nodes);
-  H.d_fg_node.upload(H.fg_node);
-  H.d_fg_tile.upload(H.fg_tile);
   H.d_bg_node.upload(H.bg_node);
   H.d_bg_tile.upload(H.bg_tile);
+  H.d_fs_node.upload(H.fs_node);
+  H.d_fs_tile.upload(H.fs_tile);
+  H.d_bs_node.upload(H.bs_node);
+  H.d_bs_tile.upload(H.bs_tile);
+  H.d_pinvoff.upload(H.pinvoff);
+  H.d_inv_node.upload(H.inv_node);
+  H.d_inv_panel.upload(H.inv_panel);
+  H.d_pinv.alloc((size_t)H.pinv_total);
   H.y_nrhs = 0;
   H.d_F.alloc(((size_t)H.f_total * H.esize + 7) / 8);
   H.d_dv.alloc((size_t)H.i_total);
@@ -2332,6 +2588,8 @@ static DevPlan dev_plan(Handle& H) {
   P.ptype = H.d_ptype.p;
   P.W = H.d_W.p;
   P.U = H.d_U.p;
+  P.pinv = H.d_pinv.p;
+  P.pinvoff = H.d_pinvoff.p;
   P.vals = H.d_vals.p;
   P.diag = H.has_diag ? H.d_diag.p : nullptr;
   P.ustab = H.opt.ustab;
@@ -2491,6 +2749,11 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       }
     }
   }
+  if (!H.inv_node.empty()) {
+    ProfScope ps_(H, st, KC_INVERT);
+    hipLaunchKernelGGL(k_panel_inv<T>, dim3((unsigned)H.inv_node.size()), dim3(kWave), 0, st, P,
+                       H.d_inv_node.p, H.d_inv_panel.p);
+  }
   HIPCHECK(hipGetLastError());
 }
 
@@ -2609,11 +2872,10 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   P.lds_t = lds_t;
   size_t lds = sizeof(double) * ((size_t)lds_t + (size_t)kSB * kSB);
   if (lds > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
-  const size_t lds_fg = sizeof(double) * ((size_t)lds_t + 4 * kWave);
   const size_t lds_bg = sizeof(double) * (size_t)H.max_cap;
   const long long ys = H.i_total, us = H.u_total;
   double* Y = H.d_Y.p;
-  constexpr int kBigThreads = 1024;
+  constexpr int kGatherThreads = 1024;
   for (int l = 0; l < nlev; l++) {
     ProfScope ps_(H, st, KC_FORWARD);
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
@@ -2622,11 +2884,15 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
                          H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol, Y, ys);
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
-      hipLaunchKernelGGL((k_forward<T, kBigThreads, true>), dim3(g1 - g0), dim3(kBigThreads), lds,
-                         st, P, H.d_large_nodes.p + g0, dx, ldx, nrhs, us, chol, Y, ys);
-      const int f0 = H.fg_ptr[l], f1 = H.fg_ptr[l + 1];
-      hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(f1 - f0), dim3(kTPB), lds_fg, st, P,
-                         H.d_fg_node.p + f0, H.d_fg_tile.p + f0, nrhs, us, Y, ys);
+      hipLaunchKernelGGL((k_forward<T, kGatherThreads, true>), dim3(g1 - g0),
+                         dim3(kGatherThreads), sizeof(double) * (size_t)lds_t, st, P,
+                         H.d_large_nodes.p + g0, dx, ldx, nrhs, us, chol, Y, ys);
+      for (int t = 0; t < H.fs_lvl[l + 1] - H.fs_lvl[l]; t++) {
+        const int slot = H.fs_lvl[l] + t;
+        const int f0 = H.fs_ptr[slot], f1 = H.fs_ptr[slot + 1];
+        hipLaunchKernelGGL(k_fwd_step<T>, dim3(f1 - f0), dim3(kTPB), 0, st, P,
+                           H.d_fs_node.p + f0, H.d_fs_tile.p + f0, t, dx, ldx, nrhs, us, Y, ys);
+      }
     }
   }
   for (int l = nlev - 1; l >= 0; l--) {
@@ -2636,13 +2902,17 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
       const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
       hipLaunchKernelGGL(k_bwd_gemv<T>, dim3(b1 - b0), dim3(kTPB), lds_bg, st, P,
                          H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys);
-      hipLaunchKernelGGL((k_backward<T, kBigThreads, true>), dim3(g1 - g0), dim3(kBigThreads),
-                         lds, st, P, H.d_large_nodes.p + g0, dx, ldx, nrhs, chol, Y, ys);
+      for (int t = 0; t < H.bs_lvl[l + 1] - H.bs_lvl[l]; t++) {
+        const int slot = H.bs_lvl[l] + t;
+        const int f0 = H.bs_ptr[slot], f1 = H.bs_ptr[slot + 1];
+        hipLaunchKernelGGL(k_bwd_step<T>, dim3(f1 - f0), dim3(kTPB), 0, st, P,
+                           H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
+      }
     }
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
     if (s1 > s0)
-      hipLaunchKernelGGL((k_backward<T, kTPB, false>), dim3(s1 - s0), dim3(kTPB), lds, st, P,
-                         H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol, Y, ys);
+      hipLaunchKernelGGL((k_backward<T, kTPB>), dim3(s1 - s0), dim3(kTPB), lds, st, P,
+                         H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol);
   }
   HIPCHECK(hipGetLastError());
 }

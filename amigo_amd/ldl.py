@@ -97,7 +97,7 @@ EXPORTS = (
 )
 
 KERNEL_CLASSES = ("prepare", "assemble", "front_lds", "factor_small", "panel",
-                  "update", "forward", "backward", "residual")
+                  "update", "forward", "backward", "residual", "panel_inverse")
 
 _lib = None
 _i32p = ctypes.POINTER(ctypes.c_int)

@@ -22,7 +22,8 @@ typedef struct kkt_asm_handle* kkt_asm_t;
  * include/cuda/component_group_backend.cuh:236-250).  fixed[nfixed] lists
  * fixed variables: their rows and columns become 0 with 1 on the diagonal
  * (FixedVariables::zero_rows_and_columns, include/fixed_variables.h:66-69,
- * 88-130).  For an element group of ncomp variables with indices idx[e][.],
+ * 88-130); indices outside [0, n) are ignored, as FixedVariables ignores
+ * them (include/fixed_variables.h:36-41).  For an element group of ncomp variables with indices idx[e][.],
  * the reference's contribution order is q = ncomp*(ncomp*e + k) + i for the
  * entry (idx[e][k], idx[e][i]) (include/cuda/component_group_backend.cuh:
  * 159-189, include/component_group.h:491-528). */

@@ -129,7 +129,8 @@ int kkt_ldl_last_times(kkt_ldl_t h, double* factor_ms, double* solve_ms);
  * factor / solve is bracketed by hipEvents on the caller's stream (this adds
  * launch latency; use it for attribution, not for the headline time).
  * Classes: 0 prepare, 1 assembly, 2 LDS fronts, 3 small fronts, 4 panel,
- * 5 MFMA trailing update, 6 forward solve, 7 backward solve, 8 residual.
+ * 5 MFMA trailing update, 6 forward solve, 7 backward solve, 8 residual,
+ * 9 panel inverses of the large fronts (after the factorization).
  * No counterpart in the reference, which has no kernel instrumentation
  * (amigo/algorithm/iteration_logger.py:28-39 only timestamps iterations). */
 int kkt_ldl_set_profiling(kkt_ldl_t h, int on);

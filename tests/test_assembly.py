@@ -141,6 +141,39 @@ def test_oracle_matches_compiled_reference_random(seed):
 
 # --------------------------------------------------------- C ABI (host) --
 
+def _c_create(crow, ccol, fixed):
+    import ctypes
+    from amigo_amd import assembly as A
+    lib = A._lib()
+    rowp = np.array([0, 2, 4], np.int32)
+    cols = np.array([0, 1, 0, 1], np.int32)
+    crow = np.asarray(crow, np.int32)
+    ccol = np.asarray(ccol, np.int32)
+    fx = np.asarray(fixed, np.int32)
+    h = ctypes.c_void_p()
+    st = lib.kkt_asm_create(2, rowp.ctypes.data_as(A._i32p), cols.ctypes.data_as(A._i32p),
+                            len(crow), crow.ctypes.data_as(A._i32p),
+                            ccol.ctypes.data_as(A._i32p), len(fx), fx.ctypes.data_as(A._i32p),
+                            ctypes.byref(h))
+    return lib, st, h
+
+
+def test_c_abi_rejects_contributions_out_of_range():
+    lib, st, _ = _c_create([0, 1], [0, 5], [])
+    assert st < 0 and b"out of range" in lib.kkt_asm_last_error()
+
+
+def test_c_abi_ignores_fixed_indices_out_of_range():
+    """As FixedVariables ignores them: pattern set-up passes validation (and,
+    without a device, fails only at the first HIP allocation)."""
+    lib, st, h = _c_create([0, 1], [0, 1], [-4, 9, 1])
+    if has_gpu():
+        assert st == 0
+        lib.kkt_asm_destroy(h)
+    else:
+        assert st < 0 and b"HIP" in lib.kkt_asm_last_error()
+
+
 def test_create_checks_arguments_without_a_gpu():
     from amigo_amd.assembly import KKTAssembler
     rowp = np.array([0, 2, 4], np.int32)
