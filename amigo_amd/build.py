@@ -2,9 +2,12 @@
 
     python -m amigo_amd.build
 
-Compiles amigo_amd/csrc/{symbolic.cpp,kkt_ldl.hip,assembly.hip} with hipcc into
-amigo_amd/libkktldl.so.  Every HIP translation unit is built with packed
-FP32 arithmetic disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+Compiles amigo_amd/csrc/{symbolic.cpp,kkt_ldl.hip,assembly.hip,ipm.hip} with
+hipcc into amigo_amd/libkktldl.so.  Every HIP translation unit is built with
+packed FP32 arithmetic disabled (-Xclang -target-feature -Xclang
+-packed-fp32-ops).  ipm.hip is built without floating-point contraction, so
+that its elementwise results are the reference's expressions evaluated
+operation by operation.
 """
 
 import os
@@ -34,9 +37,11 @@ def build(verbose=True):
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     objs = []
-    for src in ("symbolic.cpp", "kkt_ldl.hip", "assembly.hip"):
+    for src in ("symbolic.cpp", "kkt_ldl.hip", "assembly.hip", "ipm.hip"):
         obj = os.path.join(build_dir, src + ".o")
         extra = ["-x", "hip"] if src.endswith(".hip") else []
+        if src == "ipm.hip":
+            extra += ["-ffp-contract=off"]
         _run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"

@@ -103,6 +103,7 @@ struct DevPlan {
   const int* am_row;   // nominal row in the front
   const int* am_col;   // own column of the entry inside its front
   int lds_t;           // solve kernels: doubles of LDS before the staged block
+  int lds_b;           // solve kernels: leading dimension of the staged block
   // dynamic per node
   int* din;            // delayed pivots received
   int* m;              // actual front dimension
@@ -1574,6 +1575,7 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
 // block held in registers, the rows below by the whole workgroup.
 // --------------------------------------------------------------------------
 constexpr int kSB = 64;
+constexpr int kTinyFront = 64;  // fronts up to this dimension: one wave each
 
 // End of the solve block starting at jb: kSB columns, shortened so that a
 // 2x2 pivot pair is never split.
@@ -1649,10 +1651,11 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
     for (int jb = 0; jb < npiv;) {
       const int je = block_end(ptype, jb, npiv, chol);
       const int bs = je - jb;
-      double* Lb = t + P.lds_t;  // kSB x kSB block, column major
+      const int ldb = P.lds_b;
+      double* Lb = t + P.lds_t;  // bs x bs block, column major, leading dimension ldb
       for (int e = tid; e < bs * bs; e += NTH) {
         const int c = e / bs, rr = e - c * bs;
-        Lb[rr + c * kSB] = (double)L[jb + rr + (size_t)(jb + c) * m];
+        Lb[rr + c * ldb] = (double)L[jb + rr + (size_t)(jb + c) * m];
       }
       __syncthreads();
       if (w == 0) {
@@ -1666,11 +1669,11 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
           if (ty == 1) {
             if (chol && lane == c) tr = tr / dr;
             const double tc = lane_d(tr, c);
-            if (lane > c && lane < bs) tr -= Lb[lane + c * kSB] * tc;
+            if (lane > c && lane < bs) tr -= Lb[lane + c * ldb] * tc;
           } else if (ty == 2) {
             const double ta = lane_d(tr, c), tb = lane_d(tr, c + 1);
             if (lane > c + 1 && lane < bs)
-              tr -= Lb[lane + c * kSB] * ta + Lb[lane + (c + 1) * kSB] * tb;
+              tr -= Lb[lane + c * ldb] * ta + Lb[lane + (c + 1) * ldb] * tb;
           }
         }
         if (lane < bs) t[row] = tr;
@@ -1749,10 +1752,11 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
         }
         __syncthreads();
       }
+      const int ldb = P.lds_b;
       double* Lb = t + P.lds_t;
       for (int e = tid; e < bs * bs; e += NTH) {
         const int c = e / bs, rr = e - c * bs;
-        Lb[rr + c * kSB] = (double)L[jb + rr + (size_t)(jb + c) * m];
+        Lb[rr + c * ldb] = (double)L[jb + rr + (size_t)(jb + c) * m];
       }
       __syncthreads();
       if (w == 0) {
@@ -1767,11 +1771,11 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
           if (ty == 1) {
             if (lane == i) tv = chol ? (tv - acc) / dc : tv - acc;
             const double ti = lane_d(tv, i);
-            if (lane < i) acc += Lb[i + lane * kSB] * ti;
+            if (lane < i) acc += Lb[i + lane * ldb] * ti;
           } else if (ty == 3) {
             if (lane == i || lane == i - 1) tv -= acc;
             const double ti = lane_d(tv, i), tj = lane_d(tv, i - 1);
-            if (lane < i - 1) acc += Lb[i + lane * kSB] * ti + Lb[i - 1 + lane * kSB] * tj;
+            if (lane < i - 1) acc += Lb[i + lane * ldb] * ti + Lb[i - 1 + lane * ldb] * tj;
           }
         }
         if (lane < bs) t[col] = tv;
@@ -1855,8 +1859,9 @@ __global__ void __launch_bounds__(kWave) k_panel_inv(DevPlan P, const int* inode
 //   x_s = B_s^{-1} y[b0, b1)                         (lanes = rows of x_s)
 //   y/U[i] -= L[i, b0:b1] x_s  for its kStepRows rows i >= b1
 // Tile 0 also applies D^{-1} to x_s and writes the panel's entries of x.
-// A 2x2 pivot split by the panel boundary is finished by the next step,
-// which recomputes the first pivot's entry from the previous panel.
+// A 2x2 pivot (b1 - 1, b1) split by the boundary is finished by tile 0 of
+// the same step: B_{s+1} is unit lower triangular, so the solved entry of
+// row b1 is its y after this step's update, which tile 0 computes.
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
                                                    const int* wtile, int s, double* x,
@@ -1914,13 +1919,27 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
     for (int q = 0; q < CQ; q++) g += lrow[q] * xp[wv * CQ + q];
     part[wv][lane] = g;
     __syncthreads();
+    double* xr = x + (size_t)r * ldx;
     if (wv == 0 && i < m) {
       const double sum = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-      if (i < npiv) y[i] -= sum;
-      else P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)] -= sum;
+      if (i < npiv) {
+        const double yn = y[i] - sum;
+        y[i] = yn;
+        if (cut && tile == 0 && lane == 0) {  // i == b1: the pair (b1 - 1, b1)
+          const int j = b1 - 1;
+          const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
+          const double dinv = 1.0 / (d11 * d22 - d21 * d21);
+          const double ta = xp[w - 1], tb = yn;
+          xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
+          xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
+        }
+      } else {
+        P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)] -= sum;
+      }
     }
     if (tile == 0 && wv == 1 && lane < w) {
-      double* xr = x + (size_t)r * ldx;
+      // 1x1 pivots and 2x2 pivots inside the panel; the split pairs are
+      // written above (second pivot first in the panel: by the step before)
       const int j = b0 + lane;
       const int ty = ptype[j];
       if (ty == 1) {
@@ -1931,17 +1950,6 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
         const double ta = xp[lane], tb = xp[lane + 1];
         xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
         xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
-      } else if (ty == 3 && lane == 0 && s > 0) {
-        // first pivot of the pair is the last row of panel s-1
-        const double* invp = inv - kPinvPanel;
-        const double* yp = y + (b0 - kPB);
-        double ta = 0.0;
-        for (int c = 0; c < kPB; c++) ta += invp[(kPB - 1) + c * kPB] * yp[c];
-        const double d11 = dv[j - 1], d21 = dv2[j - 1], d22 = dv[j];
-        const double dinv = 1.0 / (d11 * d22 - d21 * d21);
-        const double tb = xp[0];
-        xr[unpiv(vars[j - 1])] = dinv * (d22 * ta - d21 * tb);
-        xr[unpiv(vars[j])] = dinv * (d11 * tb - d21 * ta);
       }
     }
     __syncthreads();
@@ -2197,6 +2205,9 @@ struct Handle {
   // slot of level l, a slot's work items are [fs_ptr[slot], fs_ptr[slot+1]));
   // Y holds a large node's pivot vector
   std::vector<int> sv_ptr, sv_nodes, bg_ptr, bg_node, bg_tile;
+  // sv nodes of a level: the first sv_tiny[l] have cap <= kTinyFront and are
+  // solved by one wave each; LDS shape per (level, group): sv_lt, sv_lb
+  std::vector<int> sv_tiny, sv_lt, sv_lb;
   std::vector<int> fs_lvl, fs_ptr, fs_node, fs_tile, bs_lvl, bs_ptr, bs_node, bs_tile;
   DBuf<int> d_sv_nodes, d_bg_node, d_bg_tile, d_fs_node, d_fs_tile, d_bs_node, d_bs_tile;
   DBuf<double> d_Y;
@@ -2204,6 +2215,7 @@ struct Handle {
   std::vector<long long> pinvoff;
   std::vector<int> inv_node, inv_panel;
   long long pinv_total = 0;
+  std::vector<int> lvl_steps;  // per level: panel steps needed by the last factorization
   DBuf<long long> d_pinvoff;
   DBuf<int> d_inv_node, d_inv_panel;
   DBuf<double> d_pinv;
@@ -2349,6 +2361,9 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   }
   H.upd_lvl[nlev] = (int)H.upd_ptr.size() - 1;
   H.sv_ptr.assign(nlev + 1, 0);
+  H.sv_tiny.assign(nlev, 0);
+  H.sv_lt.assign(2 * (size_t)nlev, 0);
+  H.sv_lb.assign(2 * (size_t)nlev, 0);
   H.bg_ptr.assign(nlev + 1, 0);
   H.sv_nodes.clear();
   H.bg_node.clear();
@@ -2388,6 +2403,17 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     }
     H.sv_ptr[l + 1] = (int)H.sv_nodes.size();
     H.bg_ptr[l + 1] = (int)H.bg_node.size();
+    // tiny fronts (cap <= kTinyFront) lead the level's list (LDS fronts are
+    // sorted by cap); each group gets LDS for its largest front vector and
+    // its largest diagonal block
+    for (int q = H.sv_ptr[l]; q < H.sv_ptr[l + 1]; q++) {
+      const int k = H.sv_nodes[q];
+      const bool tiny = H.cap[k] <= kTinyFront && q - H.sv_ptr[l] == H.sv_tiny[l];
+      if (tiny) H.sv_tiny[l]++;
+      const int g = 2 * l + (tiny ? 0 : 1);
+      H.sv_lt[g] = std::max(H.sv_lt[g], (H.cap[k] + 7) & ~7);
+      H.sv_lb[g] = std::max(H.sv_lb[g], std::min(kSB, H.cap[k] - H.cs[k]));
+    }
     // forward step t: rows [b1, m) with b1 > t*kPB; tile 0 always exists
     H.fs_lvl[l] = (int)H.fs_ptr.size() - 1;
     for (int t = 0; t < maxp; t++) {
@@ -2837,6 +2863,12 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
   HIPCHECK(hipMemcpy(H.h_npiv.data(), H.d_npiv.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(H.h_ndel.data(), H.d_ndel.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(H.h_cnt.data(), H.d_cnt.p, sizeof(int) * 3 * H.N, hipMemcpyDeviceToHost));
+  // the solve launches only the panel steps that hold pivots
+  const int nlev = (int)H.lvl_ptr.size() - 1;
+  H.lvl_steps.assign(nlev, 0);
+  for (int l = 0; l < nlev; l++)
+    for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++)
+      H.lvl_steps[l] = std::max(H.lvl_steps[l], (H.h_npiv[H.large_nodes[q]] + kPB - 1) / kPB);
   int nd = 0, n2 = 0;
   long long stored = 0;
   for (int k = 0; k < H.N; k++) {
@@ -2870,24 +2902,41 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   const int chol = H.opt.solver_type == KKT_LDL_CHOLESKY;
   const int lds_t = (H.max_cap + 31) & ~31;
   P.lds_t = lds_t;
-  size_t lds = sizeof(double) * ((size_t)lds_t + (size_t)kSB * kSB);
-  if (lds > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
+  P.lds_b = kSB;
+  // per (level, group) plan copies with their LDS shape
+  auto sv_plan = [&](int g, size_t* bytes) {
+    DevPlan Q = P;
+    Q.lds_t = H.sv_lt[g];
+    Q.lds_b = std::max(1, H.sv_lb[g]);
+    *bytes = sizeof(double) * ((size_t)Q.lds_t + (size_t)Q.lds_b * Q.lds_b);
+    if (*bytes > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
+    return Q;
+  };
+  size_t lds = 0;
   const size_t lds_bg = sizeof(double) * (size_t)H.max_cap;
   const long long ys = H.i_total, us = H.u_total;
   double* Y = H.d_Y.p;
   constexpr int kGatherThreads = 1024;
   for (int l = 0; l < nlev; l++) {
     ProfScope ps_(H, st, KC_FORWARD);
-    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
-    if (s1 > s0)
-      hipLaunchKernelGGL((k_forward<T, kTPB, false>), dim3(s1 - s0), dim3(kTPB), lds, st, P,
+    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
+    if (sm > s0) {
+      const DevPlan Q = sv_plan(2 * l, &lds);
+      hipLaunchKernelGGL((k_forward<T, kWave, false>), dim3(sm - s0), dim3(kWave), lds, st, Q,
                          H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol, Y, ys);
+    }
+    if (s1 > sm) {
+      const DevPlan Q = sv_plan(2 * l + 1, &lds);
+      hipLaunchKernelGGL((k_forward<T, kTPB, false>), dim3(s1 - sm), dim3(kTPB), lds, st, Q,
+                         H.d_sv_nodes.p + sm, dx, ldx, nrhs, us, chol, Y, ys);
+    }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
       hipLaunchKernelGGL((k_forward<T, kGatherThreads, true>), dim3(g1 - g0),
                          dim3(kGatherThreads), sizeof(double) * (size_t)lds_t, st, P,
                          H.d_large_nodes.p + g0, dx, ldx, nrhs, us, chol, Y, ys);
-      for (int t = 0; t < H.fs_lvl[l + 1] - H.fs_lvl[l]; t++) {
+      const int nst = std::min(H.fs_lvl[l + 1] - H.fs_lvl[l], H.lvl_steps[l]);
+      for (int t = 0; t < nst; t++) {
         const int slot = H.fs_lvl[l] + t;
         const int f0 = H.fs_ptr[slot], f1 = H.fs_ptr[slot + 1];
         hipLaunchKernelGGL(k_fwd_step<T>, dim3(f1 - f0), dim3(kTPB), 0, st, P,
@@ -2902,17 +2951,25 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
       const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
       hipLaunchKernelGGL(k_bwd_gemv<T>, dim3(b1 - b0), dim3(kTPB), lds_bg, st, P,
                          H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys);
-      for (int t = 0; t < H.bs_lvl[l + 1] - H.bs_lvl[l]; t++) {
+      const int nst = std::min(H.bs_lvl[l + 1] - H.bs_lvl[l], H.lvl_steps[l]);
+      for (int t = 0; t < nst; t++) {
         const int slot = H.bs_lvl[l] + t;
         const int f0 = H.bs_ptr[slot], f1 = H.bs_ptr[slot + 1];
         hipLaunchKernelGGL(k_bwd_step<T>, dim3(f1 - f0), dim3(kTPB), 0, st, P,
                            H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
       }
     }
-    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
-    if (s1 > s0)
-      hipLaunchKernelGGL((k_backward<T, kTPB>), dim3(s1 - s0), dim3(kTPB), lds, st, P,
+    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
+    if (sm > s0) {
+      const DevPlan Q = sv_plan(2 * l, &lds);
+      hipLaunchKernelGGL((k_backward<T, kWave>), dim3(sm - s0), dim3(kWave), lds, st, Q,
                          H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol);
+    }
+    if (s1 > sm) {
+      const DevPlan Q = sv_plan(2 * l + 1, &lds);
+      hipLaunchKernelGGL((k_backward<T, kTPB>), dim3(s1 - sm), dim3(kTPB), lds, st, Q,
+                         H.d_sv_nodes.p + sm, dx, ldx, nrhs, chol);
+    }
   }
   HIPCHECK(hipGetLastError());
 }

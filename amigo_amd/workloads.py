@@ -264,6 +264,48 @@ def poisson2d_elements(N=256, alpha=1e-4):
     return dict(n=P.n, rowp=P.rowp, cols=P.cols, idx=idx, H=H, fixed=fixed, vals=P.vals)
 
 
+def poisson2d_ipm_state(N=256, seed=0, mu=1e-6, active_frac=0.2):
+    """An interior-point iterate of :func:`poisson2d`'s problem in the
+    reference's OptProblemInfo layout (include/interior_point_backend.h:54-62):
+    the primals are the free states y (no bounds) and the controls u in
+    [-1, 1]; the constraints are the free adjoints lambda (target 0); the
+    fixed boundary y and lambda are neither.  u sits strictly inside its
+    bounds with a fraction nearly active, the bound duals on the central path
+    (z = mu / gap), and grad, the primal step and the dual steps are seeded
+    normals.
+
+    Returns dict(n, pidx, lbx, ubx, cidx, lbh, mu, x, zl, zu, grad, dx, dzl,
+    dzu)."""
+    rng = np.random.default_rng(seed + 17)
+    nn = (N + 1) ** 2
+    n = 3 * nn
+    gi, gj = np.divmod(np.arange(nn), N + 1)
+    bnd = (gi == 0) | (gj == 0) | (gi == N) | (gj == N)
+    y = 3 * np.arange(nn)[~bnd]
+    u = 3 * np.arange(nn) + 1
+    lam = 3 * np.arange(nn)[~bnd] + 2
+    pidx = np.sort(np.concatenate([y, u])).astype(np.int32)
+    isu = (pidx % 3) == 1
+    lbx = np.where(isu, -1.0, -np.inf)
+    ubx = np.where(isu, 1.0, np.inf)
+    x = rng.standard_normal(n)
+    xu = -1.0 + 2.0 * (0.05 + 0.9 * rng.random(len(u)))
+    act = rng.random(len(u)) < active_frac
+    side = rng.random(len(u)) < 0.5
+    gap = 10.0 ** rng.uniform(-6, -2, size=len(u))
+    xu = np.where(act & side, -1.0 + gap, xu)
+    xu = np.where(act & ~side, 1.0 - gap, xu)
+    x[u] = xu
+    xp = x[pidx]
+    zl = np.where(isu, mu / (xp - lbx), 0.0)
+    zu = np.where(isu, mu / (ubx - xp), 0.0)
+    grad = rng.standard_normal(n)
+    return dict(n=n, pidx=pidx, lbx=lbx, ubx=ubx, cidx=lam.astype(np.int32),
+                lbh=np.zeros(len(lam)), mu=mu, x=x, zl=zl, zu=zu, grad=grad,
+                dx=rng.standard_normal(n), dzl=rng.standard_normal(len(pidx)),
+                dzu=rng.standard_normal(len(pidx)))
+
+
 def _cartpole_derivs(q, u, w, g=9.81, L0=0.5, m1=1.0, m2=0.3):
     """f(q,u), df/d(q,u) and the Hessian of w^T f for every node (torch)."""
     import torch

@@ -267,6 +267,63 @@ def bench_assembly(args, torch, reps=50, cpu=True):
     return res
 
 
+def bench_condense(args, torch, reps=50, cpu=True):
+    """The IPM condensation around the solve on the same problem's iterate
+    (mesh 256: 198,147 entries, 131,074 primals of which the 66,049 controls
+    are bounded, 65,025 constraints): the barrier
+    diagonal Sigma (k_sigma) and the condensed right-hand side with its norm
+    (k_residual + final reduction), device time per call over reps calls
+    (HIP events on torch's current stream, the one the launches use), and
+    the oracle's numpy restatement of the reference's loops on the host."""
+    from amigo_amd import workloads as W
+    from amigo_amd.ipm import InteriorPointKernels
+
+    S = W.poisson2d_ipm_state(N=args.mesh)
+    K = InteriorPointKernels(S["n"], S["pidx"], S["lbx"], S["ubx"], S["cidx"], S["lbh"])
+    d = {k: torch.tensor(S[k], device="cuda") for k in ("x", "zl", "zu", "grad")}
+    st = (d["x"], d["zl"], d["zu"])
+    diag = torch.empty(S["n"], dtype=torch.float64, device="cuda")
+    res = torch.empty_like(diag)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for _ in range(3):
+        K.compute_diagonal(st, diag)
+        K.compute_residual(S["mu"], st, d["grad"], res)
+    ev[0].record()
+    for _ in range(reps):
+        K.compute_diagonal(st, diag)
+    ev[1].record()
+    ev[2].record()
+    for _ in range(reps):
+        K.compute_residual(S["mu"], st, d["grad"], res)
+    ev[3].record()
+    torch.cuda.synchronize()
+    ms_sig = ev[0].elapsed_time(ev[1]) / reps
+    ms_res = ev[2].elapsed_time(ev[3]) / reps
+    n, npr = S["n"], len(S["pidx"])
+    # per entry: the two index maps (8 B) and the output (8 B); per primal:
+    # x, lb, ub, zl, zu (40 B); the residual also reads grad (8 B / entry)
+    # and the constraint targets (8 B / constraint)
+    b_sig = 16.0 * n + 40.0 * npr
+    b_res = b_sig + 8.0 * n + 8.0 * len(S["cidx"])
+    out = dict(kernels=["k_sigma", "k_residual"], ms_sigma=round(ms_sig, 5),
+               ms_residual=round(ms_res, 5), calls=reps, n=n, primals=npr,
+               algorithmic_bytes_sigma=b_sig, algorithmic_bytes_residual=b_res,
+               sigma_GBs=round(b_sig / (ms_sig * 1e-3) / 1e9, 1),
+               residual_GBs_incl_sync=round(b_res / (ms_res * 1e-3) / 1e9, 1))
+    if cpu:
+        from oracle import ipm_condense as O
+        tc = []
+        for _ in range(3):
+            t = time.perf_counter()
+            O.compute_diagonal(n, S["pidx"], S["lbx"], S["ubx"], S["x"], S["zl"], S["zu"])
+            O.compute_residual(n, S["pidx"], S["lbx"], S["ubx"], S["cidx"], S["lbh"], S["mu"],
+                               S["x"], S["zl"], S["zu"], S["grad"])
+            tc.append((time.perf_counter() - t) * 1e3)
+        out["cpu_oracle_ms"] = round(float(np.median(tc)), 3)
+        out["cpu_oracle_sample"] = "numpy restatement of Sigma + residual, median of 3, 1 core"
+    return out
+
+
 def cpu_baseline(P, perm, reps):
     """The reference's SparseLDL (oracle/_ref, compiled from its own headers)
     on the same KKT with the same elimination order (the matrix permuted by
@@ -346,6 +403,7 @@ def main():
     head = "fp32" if "fp32" in results else "fp64"
     r = results[head]
     asm = bench_assembly(args, torch, cpu=not args.no_cpu_baseline)
+    cond = bench_condense(args, torch, cpu=not args.no_cpu_baseline)
     rl = roofline(r, P)
     if not args.no_traffic:
         try:
@@ -389,6 +447,7 @@ def main():
         "kernels": {k: {"ms": round(t, 3), "launches": n} for k, (t, n) in r["kt"].items()},
         "roofline": rl,
         "assembly": asm,
+        "condensation": cond,
         "cpu_baseline": cpu,
     }
     if cpu and cpu.get("inertia"):
