@@ -1617,7 +1617,8 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
       s_uo[q] = P.uoff[P.ch[c0 + q]];
     }
   __syncthreads();
-  for (int r = 0; r < nrhs; r++) {
+  {
+    const int r = blockIdx.y;  // one right-hand side per grid row
     double* xr = x + (size_t)r * ldx;
     for (int i = tid; i < m; i += NTH) {
       const int pi = i < fs ? pre[i] : i;
@@ -1645,8 +1646,7 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
         if (i < npiv) y[i] = t[i];
         else U[i - npiv] = t[i];
       }
-      __syncthreads();
-      continue;
+      return;
     }
     for (int jb = 0; jb < npiv;) {
       const int je = block_end(ptype, jb, npiv, chol);
@@ -1725,7 +1725,8 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
   const double* dv = P.dv + P.ioff[p];
   const int cp = P.cptr[p];
   const int w = tid / kWave, lane = tid % kWave;
-  for (int r = 0; r < nrhs; r++) {
+  {
+    const int r = blockIdx.y;  // one right-hand side per grid row
     double* xr = x + (size_t)r * ldx;
     for (int i = tid; i < m; i += NTH) {
       int v = i < npiv ? unpiv(vars[i]) : (i < fs ? vars[i] : P.contrib_rows[cp + i - fs]);
@@ -1795,8 +1796,9 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
 // diagonal block B_s of every panel is inverted once by k_panel_inv.  B_s is
 // unit lower triangular, with the slot L[j + 1, j] of every 2x2 pivot (j,
 // j + 1) taken as zero: that slot does not belong to L (D holds the
-// coupling), also where the pair is split by a panel boundary.  A solve then walks the panels with one launch per panel
-// for all large nodes of a level: every workgroup forms x_s = B_s^{-1} y_s
+// coupling), also where the pair is split by a panel boundary.  A solve then
+// walks the panels with one launch per panel for all large nodes of a level
+// (and one grid row per right-hand side): every workgroup forms x_s = B_s^{-1} y_s
 // (forward) or B_s^{-T} y_s (backward) itself, from the kPB entries of y_s
 // and the kPB x kPB inverse, and applies x_s to its own tile of the rows
 // below the panel (forward) or of the columns before it (backward).  A
@@ -1900,7 +1902,8 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   const int* vars = P.vars + P.ioff[p];
   const double* dv = P.dv + P.ioff[p];
   const double* dv2 = P.dv2 + P.ioff[p];
-  for (int r = 0; r < nrhs; r++) {
+  {
+    const int r = blockIdx.y;  // one right-hand side per grid row
     double* y = Y + (long long)r * ystride + P.ioff[p];
     double a = 0.0;
 #pragma unroll
@@ -1999,7 +2002,8 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
     ainv[k] = i < w ? invt[lane + i * kPB] : 0.0;
   }
   const int* vars = P.vars + P.ioff[p];
-  for (int r = 0; r < nrhs; r++) {
+  {
+    const int r = blockIdx.y;  // one right-hand side per grid row
     double* y = Y + (long long)r * ystride + P.ioff[p];
     double a = 0.0;
 #pragma unroll
@@ -2048,7 +2052,8 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const T* L = front_ptr<T>(P, p);
   const int nr = m - npiv;
-  for (int r = 0; r < nrhs; r++) {
+  {
+    const int r = blockIdx.y;  // one right-hand side per grid row
     const double* xr = x + (size_t)r * ldx;
     for (int q = tid; q < nr; q += kTPB) {
       const int i = npiv + q;
@@ -2922,24 +2927,24 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
     if (sm > s0) {
       const DevPlan Q = sv_plan(2 * l, &lds);
-      hipLaunchKernelGGL((k_forward<T, kWave, false>), dim3(sm - s0), dim3(kWave), lds, st, Q,
+      hipLaunchKernelGGL((k_forward<T, kWave, false>), dim3(sm - s0, nrhs), dim3(kWave), lds, st, Q,
                          H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol, Y, ys);
     }
     if (s1 > sm) {
       const DevPlan Q = sv_plan(2 * l + 1, &lds);
-      hipLaunchKernelGGL((k_forward<T, kTPB, false>), dim3(s1 - sm), dim3(kTPB), lds, st, Q,
+      hipLaunchKernelGGL((k_forward<T, kTPB, false>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st, Q,
                          H.d_sv_nodes.p + sm, dx, ldx, nrhs, us, chol, Y, ys);
     }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
-      hipLaunchKernelGGL((k_forward<T, kGatherThreads, true>), dim3(g1 - g0),
+      hipLaunchKernelGGL((k_forward<T, kGatherThreads, true>), dim3(g1 - g0, nrhs),
                          dim3(kGatherThreads), sizeof(double) * (size_t)lds_t, st, P,
                          H.d_large_nodes.p + g0, dx, ldx, nrhs, us, chol, Y, ys);
       const int nst = std::min(H.fs_lvl[l + 1] - H.fs_lvl[l], H.lvl_steps[l]);
       for (int t = 0; t < nst; t++) {
         const int slot = H.fs_lvl[l] + t;
         const int f0 = H.fs_ptr[slot], f1 = H.fs_ptr[slot + 1];
-        hipLaunchKernelGGL(k_fwd_step<T>, dim3(f1 - f0), dim3(kTPB), 0, st, P,
+        hipLaunchKernelGGL(k_fwd_step<T>, dim3(f1 - f0, nrhs), dim3(kTPB), 0, st, P,
                            H.d_fs_node.p + f0, H.d_fs_tile.p + f0, t, dx, ldx, nrhs, us, Y, ys);
       }
     }
@@ -2949,25 +2954,25 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
       const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
-      hipLaunchKernelGGL(k_bwd_gemv<T>, dim3(b1 - b0), dim3(kTPB), lds_bg, st, P,
+      hipLaunchKernelGGL(k_bwd_gemv<T>, dim3(b1 - b0, nrhs), dim3(kTPB), lds_bg, st, P,
                          H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys);
       const int nst = std::min(H.bs_lvl[l + 1] - H.bs_lvl[l], H.lvl_steps[l]);
       for (int t = 0; t < nst; t++) {
         const int slot = H.bs_lvl[l] + t;
         const int f0 = H.bs_ptr[slot], f1 = H.bs_ptr[slot + 1];
-        hipLaunchKernelGGL(k_bwd_step<T>, dim3(f1 - f0), dim3(kTPB), 0, st, P,
+        hipLaunchKernelGGL(k_bwd_step<T>, dim3(f1 - f0, nrhs), dim3(kTPB), 0, st, P,
                            H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
       }
     }
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
     if (sm > s0) {
       const DevPlan Q = sv_plan(2 * l, &lds);
-      hipLaunchKernelGGL((k_backward<T, kWave>), dim3(sm - s0), dim3(kWave), lds, st, Q,
+      hipLaunchKernelGGL((k_backward<T, kWave>), dim3(sm - s0, nrhs), dim3(kWave), lds, st, Q,
                          H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol);
     }
     if (s1 > sm) {
       const DevPlan Q = sv_plan(2 * l + 1, &lds);
-      hipLaunchKernelGGL((k_backward<T, kTPB>), dim3(s1 - sm), dim3(kTPB), lds, st, Q,
+      hipLaunchKernelGGL((k_backward<T, kTPB>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st, Q,
                          H.d_sv_nodes.p + sm, dx, ldx, nrhs, chol);
     }
   }
