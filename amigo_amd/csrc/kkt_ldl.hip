@@ -28,6 +28,7 @@
 //  * Solve: multifrontal forward pass (front vectors pulled from the
 //    children's update vectors, same maps as the assembly) and a backward
 //    pass that reads ancestors' solution entries; level scheduled.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -2647,13 +2648,18 @@ static std::vector<int> initial_slack(const Handle& H) {
   return slack;
 }
 
-// Record an event pair around a launch when profiling is on.
+// An event pair per launch (or group of launches) when profiling is on.
+// By default the pair brackets the launches on the stream, which includes
+// their dispatch.  With kernel = true the pair is handed to one
+// hipExtLaunchKernelGGL (start(), stop()), which stamps the kernel's own
+// start and end, as a kernel trace measures it.
 struct ProfScope {
   Handle& H;
   hipStream_t st;
-  bool on;
+  bool on, kernel;
   size_t slot = 0;
-  ProfScope(Handle& h, hipStream_t s, int cls) : H(h), st(s), on(h.profiling) {
+  ProfScope(Handle& h, hipStream_t s, int cls, bool kernel_only = false)
+      : H(h), st(s), on(h.profiling), kernel(kernel_only) {
     if (!on) return;
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
@@ -2662,10 +2668,12 @@ struct ProfScope {
     H.prof_ev.push_back(b);
     H.prof_cls.push_back(cls);
     slot = H.prof_ev.size() - 2;
-    HIPCHECK(hipEventRecord(a, st));
+    if (!kernel) HIPCHECK(hipEventRecord(a, st));
   }
+  hipEvent_t start() const { return on ? H.prof_ev[slot] : nullptr; }
+  hipEvent_t stop() const { return on ? H.prof_ev[slot + 1] : nullptr; }
   ~ProfScope() {
-    if (on) (void)hipEventRecord(H.prof_ev[slot + 1], st);
+    if (on && !kernel) (void)hipEventRecord(H.prof_ev[slot + 1], st);
   }
 };
 
@@ -2704,19 +2712,20 @@ static size_t lds_front_bytes(int c, size_t esize) {
 // panel) when the panel's L does not fit LDS: above 2048 rows for FP32
 // fronts, above 1024 rows for FP64 fronts.
 template <class T>
-static bool launch_panel_rr(int maxcap, dim3 grid, hipStream_t st, const DevPlan& P,
-                            const int* nodes) {
+static bool launch_panel_rr(int maxcap, dim3 grid, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                            const DevPlan& P, const int* nodes) {
   constexpr int NT = 512;
   auto lds = [&](int r) {
-    return sizeof(T) * kNBLarge * (size_t)r * NT + sizeof(int) * 2 * (size_t)maxcap;
+    return (uint32_t)(sizeof(T) * kNBLarge * (size_t)r * NT + sizeof(int) * 2 * (size_t)maxcap);
   };
   if (maxcap <= NT) {
-    hipLaunchKernelGGL((k_panel_rr<NT, 1, T>), grid, dim3(NT), lds(1), st, P, nodes);
+    hipExtLaunchKernelGGL((k_panel_rr<NT, 1, T>), grid, dim3(NT), lds(1), st, e0, e1, 0, P, nodes);
   } else if (maxcap <= 2 * NT) {
-    hipLaunchKernelGGL((k_panel_rr<NT, 2, T>), grid, dim3(NT), lds(2), st, P, nodes);
+    hipExtLaunchKernelGGL((k_panel_rr<NT, 2, T>), grid, dim3(NT), lds(2), st, e0, e1, 0, P, nodes);
   } else if (maxcap <= 4 * NT && sizeof(T) == 4) {
     if constexpr (sizeof(T) == 4)
-      hipLaunchKernelGGL((k_panel_rr<NT, 4, T>), grid, dim3(NT), lds(4), st, P, nodes);
+      hipExtLaunchKernelGGL((k_panel_rr<NT, 4, T>), grid, dim3(NT), lds(4), st, e0, e1, 0, P,
+                            nodes);
   } else {
     return false;
   }
@@ -2765,18 +2774,20 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
     for (int q = g0; q < g1; q++) maxcap_large = std::max(maxcap_large, H.cap[H.large_nodes[q]]);
     for (int t = 0; t < H.large_panels[l]; t++) {
       {
-        ProfScope ps_(H, st, KC_PANEL);
-        if (!launch_panel_rr<T>(maxcap_large, dim3(g1 - g0), st, P, H.d_large_nodes.p + g0))
-          hipLaunchKernelGGL((k_panel<kPanelThreads, T>), dim3(g1 - g0), dim3(kPanelThreads),
-                             sizeof(int) * 2 * (size_t)maxcap_large, st, P,
-                             H.d_large_nodes.p + g0);
+        ProfScope ps_(H, st, KC_PANEL, true);
+        if (!launch_panel_rr<T>(maxcap_large, dim3(g1 - g0), st, ps_.start(), ps_.stop(), P,
+                                H.d_large_nodes.p + g0))
+          hipExtLaunchKernelGGL((k_panel<kPanelThreads, T>), dim3(g1 - g0), dim3(kPanelThreads),
+                                (uint32_t)(sizeof(int) * 2 * (size_t)maxcap_large), st,
+                                ps_.start(), ps_.stop(), 0, P, H.d_large_nodes.p + g0);
       }
       const int slot = H.upd_lvl[l] + t;
       const int u0 = H.upd_ptr[slot], u1 = H.upd_ptr[slot + 1];
       if (u1 > u0) {
-        ProfScope ps_(H, st, KC_UPDATE);
-        hipLaunchKernelGGL(k_update<T>, dim3(u1 - u0), dim3(kTPB), 0, st, P,
-                           H.d_upd_node.p + u0, H.d_upd_tile.p + u0, t * (kNBLarge - 2));
+        ProfScope ps_(H, st, KC_UPDATE, true);
+        hipExtLaunchKernelGGL(k_update<T>, dim3(u1 - u0), dim3(kTPB), 0, st, ps_.start(),
+                              ps_.stop(), 0, P, H.d_upd_node.p + u0, H.d_upd_tile.p + u0,
+                              t * (kNBLarge - 2));
       }
     }
   }

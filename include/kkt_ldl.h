@@ -126,8 +126,11 @@ int kkt_ldl_get_stats(kkt_ldl_t h, kkt_ldl_stats* st);
 int kkt_ldl_last_times(kkt_ldl_t h, double* factor_ms, double* solve_ms);
 
 /* Per-kernel-class device timing: when on, every launch of the next
- * factor / solve is bracketed by hipEvents on the caller's stream (this adds
- * launch latency; use it for attribution, not for the headline time).
+ * factor / solve is timed with hipEvents on the caller's stream (this adds
+ * launch latency; use it for attribution, not for the headline time).  The
+ * panel and trailing-update launches carry the events in the launch itself
+ * (hipExtLaunchKernelGGL), so their times are the kernels' own durations;
+ * the other classes are bracketed, dispatch included.
  * Classes: 0 prepare, 1 assembly, 2 LDS fronts, 3 small fronts, 4 panel,
  * 5 MFMA trailing update, 6 forward solve, 7 backward solve, 8 residual,
  * 9 panel inverses of the large fronts (after the factorization).
