@@ -13,15 +13,17 @@
 //                     contributions in child order (deterministic, no
 //                     atomics), then the node's original K entries and the
 //                     barrier diagonal are added.
-//      k_factor    -- one workgroup per front: the reference's blocked
-//                     threshold 1x1/2x2 pivoting with delayed pivots
-//                     (include/sparse_ldl.h:1316-1553), or Bunch-Kaufman for
-//                     roots (the reference calls LAPACK dsytrf there,
-//                     :1226-1248), with the W panel in LDS and an in-LDS
-//                     trailing update for small fronts.
-//      k_panel + k_update (large fronts) -- the same panel on one workgroup,
-//                     the rank-64 trailing update tiled over many
-//                     workgroups with FP64 MFMA.
+//      k_front_lds -- fronts up to kLdsFront: assembled, factored and
+//                     written back from LDS by one workgroup each, with the
+//                     reference's blocked threshold 1x1/2x2 pivoting and
+//                     delayed pivots (include/sparse_ldl.h:1316-1553), or
+//                     Bunch-Kaufman for roots (the reference calls LAPACK
+//                     dsytrf there, :1226-1248).
+//      k_panel_rr + k_update (every larger front) -- the same pivoting in
+//                     16-column panels on one workgroup with the panel's L
+//                     in LDS, and the rank-16 trailing update tiled over
+//                     many workgroups with MFMA.
+//      k_factor    -- Cholesky mode fronts above kLdsFront.
 //  * Delayed pivots make front sizes data dependent: each front slot has
 //    slack; a front that overflows marks itself and its ancestors failed,
 //    the host enlarges the slack from the observed counts and refactors.
@@ -67,7 +69,6 @@ constexpr int kNB = 64;        // panel width bound (reference block_size)
 constexpr int kTPB = 256;      // threads per workgroup
 constexpr int kWave = 64;
 constexpr int kAsmCols = 8;    // columns per assembly workgroup
-constexpr int kLargeFront = 256;  // fronts above this use k_panel + k_update
 constexpr int kPanelThreads = 1024;  // workgroup of the large-front panel
 constexpr int kNBLarge = 16;    // panel width of large fronts (narrow: the panel re-reads it per column)
 constexpr int kLdsFront = 96;   // default bound of fronts assembled and factored in LDS
@@ -2339,7 +2340,7 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
       if (H.cap[k] <= H.lds_front) {
         H.lds_nodes.push_back(k);
         H.lds_cap[l] = std::max(H.lds_cap[l], H.cap[k]);
-      } else if (H.mode[k] != 2 && H.ns[k] + H.cs[k] > kLargeFront) {
+      } else if (H.mode[k] != 2) {
         H.large_nodes.push_back(k);
         H.large_panels[l] = std::max(H.large_panels[l], (H.cap[k] - H.cs[k]) / (kNBLarge - 2) + 1);
       } else {

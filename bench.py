@@ -136,6 +136,31 @@ def run_gpu(P, precision, args, torch):
                 solve_ms=sol_ms, t_analyze_s=t_analyze, stats=stats, kt=kt)
 
 
+def bench_host_buffers(P, precision, args, reps=5):
+    """The same step with host buffers, as the reference's AmigoSolver holds
+    them (numpy values, diagonal and right-hand side): values and Sigma cross
+    PCIe at factor time, the right-hand side and the solution at solve time.
+    Median ms per step over reps steps after one warm-up."""
+    from amigo_amd.ldl import FP32_REFINE, FP64, CSRMat, OrderingType, SparseLDL
+
+    order = OrderingType.NESTED_DISSECTION if args.order == "nd" else OrderingType.AMD
+    s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals.copy()), order=order,
+                  precision=FP32_REFINE if precision == "fp32" else FP64,
+                  refine_tol=1e-10, max_refine=20)
+    x = np.empty(P.n)
+    tt = []
+    for k in range(reps + 1):
+        t0 = time.perf_counter()
+        s.factor(P.diag)
+        x[:] = P.rhs
+        s.solve(x, refine=True)
+        s.get_inertia()
+        tt.append((time.perf_counter() - t0) * 1e3)
+    res = float(np.linalg.norm(P.matrix() @ x - P.rhs) / np.linalg.norm(P.rhs))
+    return dict(precision=precision, ms_per_step=round(float(np.median(tt[1:])), 3),
+                pcie_bytes_per_step=int(8 * P.nnz + 8 * P.n + 16 * P.n), residual=res)
+
+
 def bench_multi_rhs(r, P, torch, nrhs=64, reps=3):
     """The post-optimality adjoint shape: nrhs seeded right-hand sides
     (column major, zero on the fixed rows) solved in one refined call on the
@@ -449,6 +474,7 @@ def main():
     asm = bench_assembly(args, torch, cpu=not args.no_cpu_baseline)
     cond = bench_condense(args, torch, cpu=not args.no_cpu_baseline)
     mrhs = {p: bench_multi_rhs(v, P, torch) for p, v in results.items()}
+    host = bench_host_buffers(P, head, args)
     rl = roofline(r, P)
     if not args.no_traffic:
         try:
@@ -494,6 +520,7 @@ def main():
         "assembly": asm,
         "condensation": cond,
         "multi_rhs_64": mrhs,
+        "host_buffers": host,
         "cpu_baseline": cpu,
     }
     if cpu and cpu.get("inertia"):
