@@ -1132,20 +1132,35 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
     reload();
     __syncthreads();
     int it = 0;
+    // Look-ahead: each step also forms the updates of columns k+1 and k+2 by
+    // the panel's earlier columns (la1, la2) while its reductions run, so
+    // that after a fast 1x1 / adjacent 2x2 pivot the next column is one / two
+    // FMAs away (vn); the FMA chain and its order are those of the direct
+    // update, so the values are the same bits.
+    double vn[R];
+    bool la_ok = false;
     while (kw < nb - 1 && k < nc) {
       const int par = it & 1;
       // W[:, kw]: column k updated by the panel so far
       const double* wr = sh.wc[k - kstart];
       double v[R], vp[R];
+      if (la_ok) {
 #pragma unroll
-      for (int j = 0; j < R; j++) {
-        v[j] = (double)c0[j];
-        vp[j] = 0.0;
-      }
-      for (int c = 0; c < kw; c++) {
-        const double w = wr[c];
+        for (int j = 0; j < R; j++) {
+          v[j] = vn[j];
+          vp[j] = 0.0;
+        }
+      } else {
 #pragma unroll
-        for (int j = 0; j < R; j++) v[j] = __builtin_fma(-(double)lsx(c, j), w, v[j]);
+        for (int j = 0; j < R; j++) {
+          v[j] = (double)c0[j];
+          vp[j] = 0.0;
+        }
+        for (int c = 0; c < kw; c++) {
+          const double w = wr[c];
+#pragma unroll
+          for (int j = 0; j < R; j++) v[j] = __builtin_fma(-(double)lsx(c, j), w, v[j]);
+        }
       }
       double lbest = -1.0, lrest = 0.0;
       int lidx = 0x7fffffff;
@@ -1169,6 +1184,27 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
             sh.wik[par] = x;
           }
           if (r - kstart < kWCRows) sh.wc[r - kstart][kw] = x;
+        }
+      }
+      // columns k+1, k+2 updated by the panel columns before kw (W rows of
+      // positions k+1, k+2 are cached; k + 2 - kstart <= 16 < kWCRows)
+      double la1[R], la2[R];
+      {
+        const double* w1 = sh.wc[k + 1 - kstart];
+        const double* w2 = sh.wc[k + 2 - kstart];
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          la1[j] = (double)c1[j];
+          la2[j] = (double)f2[j];
+        }
+        for (int c = 0; c < kw; c++) {
+          const double a = w1[c], b = w2[c];
+#pragma unroll
+          for (int j = 0; j < R; j++) {
+            const double l = -(double)lsx(c, j);
+            la1[j] = __builtin_fma(l, a, la1[j]);
+            la2[j] = __builtin_fma(l, b, la2[j]);
+          }
         }
       }
       wave_argmax2(lbest, lidx, lrest);
@@ -1226,11 +1262,19 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
         double lrow = 0.0;
         const double* wr2;
         if (imax == k + 1) {
-          // the imax column is the prefetched column k+1; its entry in row
-          // k is F[k+1, k], published with the column reduction
+          // the imax column is column k+1, already updated (la1); its entry
+          // in row k is F[k+1, k], published with the column reduction, with
+          // the update chain of row k
           wr2 = sh.wc[k + 1 - kstart];
 #pragma unroll
-          for (int j = 0; j < R; j++) vp[j] = (rowof(j) == k) ? sh.fk1k[par] : (double)c1[j];
+          for (int j = 0; j < R; j++) {
+            vp[j] = la1[j];
+            if (rowof(j) == k) {
+              double t = sh.fk1k[par];
+              for (int c = 0; c < kw; c++) t = __builtin_fma(-(double)lsx(c, j), wr2[c], t);
+              vp[j] = t;
+            }
+          }
         } else {
           // general imax: its W row from HBM (written by its owner), its
           // column (row imax left of the diagonal) from the front
@@ -1247,11 +1291,11 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
           }
           lds_sync();
           wr2 = sh.xw;
-        }
-        for (int c = 0; c < kw; c++) {
-          const double w = wr2[c];
+          for (int c = 0; c < kw; c++) {
+            const double w = wr2[c];
 #pragma unroll
-          for (int j = 0; j < R; j++) vp[j] = __builtin_fma(-(double)lsx(c, j), w, vp[j]);
+            for (int j = 0; j < R; j++) vp[j] = __builtin_fma(-(double)lsx(c, j), w, vp[j]);
+          }
         }
 #pragma unroll
         for (int j = 0; j < R; j++) {
@@ -1426,6 +1470,18 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
           sh.n2++;
         }
       }
+      if (fast && step == 1) {
+        const double a = sh.wc[k + 1 - kstart][kw];
+#pragma unroll
+        for (int j = 0; j < R; j++) vn[j] = __builtin_fma(-(double)lsx(kw, j), a, la1[j]);
+      } else if (fast && step == 2) {
+        const double a = sh.wc[k + 2 - kstart][kw], b = sh.wc[k + 2 - kstart][kw + 1];
+#pragma unroll
+        for (int j = 0; j < R; j++)
+          vn[j] = __builtin_fma(-(double)lsx(kw + 1, j), b,
+                                __builtin_fma(-(double)lsx(kw, j), a, la2[j]));
+      }
+      la_ok = fast;
       k += step;
       kw += step;
       if (!fast) {
@@ -2319,9 +2375,11 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     }
     H.asm_lvl_ptr[l + 1] = (int)H.asm_node.size();
   }
-  // Large fronts get the multi-workgroup panel/update path; their update
-  // tiles for panel t cover [t*(kNB-2), cap), since every panel but the last
-  // takes at least kNB-2 pivots.
+  // Large fronts get the multi-workgroup panel/update path.  A panel ends
+  // when kw >= kNBLarge-1 or the candidates run out, so every panel but a
+  // front's last takes at least kNBLarge-1 pivots: ceil(pivots / 15) panel
+  // launches suffice, and the update tiles of panel t may start at the
+  // smaller bound t*(kNBLarge-2).
   H.small_ptr.assign(nlev + 1, 0);
   H.large_ptr.assign(nlev + 1, 0);
   H.lds_ptr.assign(nlev + 1, 0);
@@ -2342,7 +2400,8 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
         H.lds_cap[l] = std::max(H.lds_cap[l], H.cap[k]);
       } else if (H.mode[k] != 2) {
         H.large_nodes.push_back(k);
-        H.large_panels[l] = std::max(H.large_panels[l], (H.cap[k] - H.cs[k]) / (kNBLarge - 2) + 1);
+        H.large_panels[l] =
+            std::max(H.large_panels[l], (H.cap[k] - H.cs[k] + kNBLarge - 2) / (kNBLarge - 1));
       } else {
         H.small_nodes.push_back(k);
       }
@@ -2719,7 +2778,12 @@ static bool launch_panel_rr(int maxcap, dim3 grid, hipStream_t st, hipEvent_t e0
   auto lds = [&](int r) {
     return (uint32_t)(sizeof(T) * kNBLarge * (size_t)r * NT + sizeof(int) * 2 * (size_t)maxcap);
   };
-  if (maxcap <= NT) {
+  if (maxcap <= NT / 2) {
+    // fronts of at most 256 rows: 4 waves, every one holding rows
+    constexpr int NH = NT / 2;
+    const uint32_t b = (uint32_t)(sizeof(T) * kNBLarge * (size_t)NH + sizeof(int) * 2 * (size_t)maxcap);
+    hipExtLaunchKernelGGL((k_panel_rr<NH, 1, T>), grid, dim3(NH), b, st, e0, e1, 0, P, nodes);
+  } else if (maxcap <= NT) {
     hipExtLaunchKernelGGL((k_panel_rr<NT, 1, T>), grid, dim3(NT), lds(1), st, e0, e1, 0, P, nodes);
   } else if (maxcap <= 2 * NT) {
     hipExtLaunchKernelGGL((k_panel_rr<NT, 2, T>), grid, dim3(NT), lds(2), st, e0, e1, 0, P, nodes);
