@@ -1854,19 +1854,24 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
 // diagonal block B_s of every panel is inverted once by k_panel_inv.  B_s is
 // unit lower triangular, with the slot L[j + 1, j] of every 2x2 pivot (j,
 // j + 1) taken as zero: that slot does not belong to L (D holds the
-// coupling), also where the pair is split by a panel boundary.  A solve then
-// walks the panels with one launch per panel for all large nodes of a level
-// (and one grid row per right-hand side): every workgroup forms x_s = B_s^{-1} y_s
-// (forward) or B_s^{-T} y_s (backward) itself, from the kPB entries of y_s
-// and the kPB x kPB inverse, and applies x_s to its own tile of the rows
-// below the panel (forward) or of the columns before it (backward).  A
-// panel step therefore costs one memory round trip spread over many CUs,
-// instead of kPB dependent column steps inside one workgroup.
+// coupling), also where the pair is split by a panel boundary.  k_panel_inv2
+// then composes the inverse of every pair of consecutive panels.  A solve
+// walks the pivots in steps of two panels (kPS columns) with one launch per
+// step for all large nodes of a level (and one grid row per right-hand
+// side): every workgroup forms x_s = B_s^{-1} y_s (forward) or B_s^{-T} y_s
+// (backward) itself, from the kPS entries of y_s and the composed inverse,
+// and applies x_s to its own tile of the rows below the step (forward) or of
+// the columns before it (backward).  A step therefore costs one memory round
+// trip spread over many CUs, instead of kPS dependent column steps inside
+// one workgroup.
 // --------------------------------------------------------------------------
 constexpr int kPB = 64;        // panel width of the partitioned inverse
 constexpr int kStepRows = 64;  // forward step: rows per workgroup (one per lane)
 constexpr int kStepCols = 32;  // backward step: columns per workgroup (8 per wave)
 constexpr long long kPinvPanel = 2LL * kPB * kPB;  // doubles per panel: B^{-1}, B^{-T}
+constexpr int kPS = 2 * kPB;   // solve step width: two panels
+// doubles per solve step: both panels' B^{-1}, B^{-T}, then X and X^T (below)
+constexpr long long kPinvPair = 3 * kPinvPanel;
 
 // One wave per (node, panel s): lane c computes column c of B_s^{-1} by
 // forward substitution against B_s held in LDS; the inverse is written
@@ -1908,54 +1913,116 @@ __global__ void __launch_bounds__(kWave) k_panel_inv(DevPlan P, const int* inode
     if (i > lane) Zs[i * (kPB + 1) + lane] = -acc;
   }
   __syncthreads();
-  double* inv = P.pinv + P.pinvoff[p] + (long long)s * kPinvPanel;
+  double* inv = P.pinv + P.pinvoff[p] + (long long)(s / 2) * kPinvPair + (s % 2) * kPinvPanel;
   for (int c = 0; c < kPB; c++) inv[lane + c * kPB] = Zs[lane * (kPB + 1) + c];
   for (int i = 0; i < kPB; i++) inv[kPB * kPB + lane + i * kPB] = Zs[i * (kPB + 1) + lane];
 }
 
-// Forward panel step s of the large nodes of a level.  Y holds the pivot
-// rows of each front vector (gathered by k_forward and updated by the
-// earlier steps), U the rows below the pivots.  Workgroup (node, tile):
-//   x_s = B_s^{-1} y[b0, b1)                         (lanes = rows of x_s)
-//   y/U[i] -= L[i, b0:b1] x_s  for its kStepRows rows i >= b1
-// Tile 0 also applies D^{-1} to x_s and writes the panel's entries of x.
-// A 2x2 pivot (b1 - 1, b1) split by the boundary is finished by tile 0 of
-// the same step: B_{s+1} is unit lower triangular, so the solved entry of
-// row b1 is its y after this step's update, which tile 0 computes.
+// Composition of two consecutive 64-column blocks into one 128-column solve
+// step: for B = [B1 0; C B2] (C = L rows of block 2, columns of block 1, with
+// the slot of a 2x2 pivot split by the block boundary taken as zero),
+// B^{-1} = [B1^{-1} 0; X B2^{-1}] with X = -B2^{-1} C B1^{-1}.  One
+// workgroup per (node, block pair) forms X and X^T after k_panel_inv.
+template <class T>
+__global__ void __launch_bounds__(kTPB) k_panel_inv2(DevPlan P, const int* inode,
+                                                     const int* ipair) {
+  __shared__ double Cs[kPB * kPB];  // C[i][j] at i + j * kPB
+  __shared__ double Ts[kPB * kPB];  // C B1^{-1}
+  const int p = inode[blockIdx.x], q = ipair[blockIdx.x];
+  if (P.stat[p] != 0) return;
+  const int m = P.m[p], npiv = P.npiv[p];
+  if (m > P.cap[p] || npiv > m) return;
+  const int ba = 2 * q * kPB, bb = ba + kPB;
+  if (bb >= npiv) return;
+  const int wb = min(kPB, npiv - bb);
+  const T* L = front_ptr<T>(P, p);
+  const int* ptype = P.ptype + P.ioff[p];
+  double* base = P.pinv + P.pinvoff[p] + (long long)q * kPinvPair;
+  const double* inva = base;
+  const double* invb = base + kPinvPanel;
+  double* X = base + 2 * kPinvPanel;
+  const bool cut = ptype[bb - 1] == 2;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kPB * kPB; e += kTPB) {
+    const int i = e % kPB, j = e / kPB;
+    double v = 0.0;
+    if (i < wb && !(cut && i == 0 && j == kPB - 1))
+      v = (double)L[(size_t)(bb + i) + (size_t)(ba + j) * m];
+    Cs[e] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < kPB * kPB; e += kTPB) {
+    const int i = e % kPB, c = e / kPB;
+    double a = 0.0;
+    for (int j = c; j < kPB; j++) a += Cs[i + j * kPB] * inva[j + c * kPB];
+    Ts[e] = a;
+  }
+  __syncthreads();
+  for (int e = tid; e < kPB * kPB; e += kTPB) {
+    const int i = e % kPB, c = e / kPB;
+    double a = 0.0;
+    for (int r = 0; r <= i; r++) a += invb[i + r * kPB] * Ts[r + c * kPB];
+    X[i + c * kPB] = -a;
+    X[kPB * kPB + c + i * kPB] = -a;
+  }
+}
+
+// Forward step s of the large nodes of a level: the 128 columns
+// [b0, b0 + 128) as two 64-blocks a, b.  Y holds the pivot rows of each
+// front vector (gathered by k_forward and updated by the earlier steps), U
+// the rows below the pivots.  Workgroup (node, tile):
+//   x_a = B_a^{-1} y_a,  x_b = X y_a + B_b^{-1} y_b          (lanes = rows)
+//   y/U[i] -= L[i, b0:b1] x  for its kStepRows rows i >= b1
+// Tile 0 also applies D^{-1} to x and writes the step's entries of x.  A
+// 2x2 pivot (b1 - 1, b1) split by the step boundary is finished by tile 0 of
+// the same step: the next step's block is unit lower triangular, so the
+// solved entry of row b1 is its y after this step's update, which tile 0
+// computes.
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
                                                    const int* wtile, int s, double* x,
                                                    int ldx, int nrhs, long long ustride,
                                                    double* Y, long long ystride) {
   constexpr int NW = kTPB / kWave;
-  constexpr int CQ = kPB / NW;  // panel columns per wave
-  __shared__ double xp[kPB];
-  __shared__ double part[NW][kWave];
+  constexpr int CQ = kPB / NW;  // block columns per wave (inverse products)
+  constexpr int CR = kPS / NW;  // step columns per wave (row update)
+  __shared__ double xp[kPS];
+  __shared__ double part[2][NW][kWave];
   const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
   const int m = P.m[p], npiv = P.npiv[p];
-  const int b0 = s * kPB;
+  const int b0 = s * kPS;
   if (b0 >= npiv) return;
-  const int b1 = min(npiv, b0 + kPB), w = b1 - b0;
+  const int b1 = min(npiv, b0 + kPS), w = b1 - b0;
+  const int wa = min(w, kPB), wb = w - wa;
   const int i0 = b1 + tile * kStepRows;
   if (tile > 0 && i0 >= m) return;
   const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
   const T* L = front_ptr<T>(P, p);
-  const double* inv = P.pinv + P.pinvoff[p] + (long long)s * kPinvPanel;
-  const int i = i0 + lane;
+  const double* base = P.pinv + P.pinvoff[p] + (long long)s * kPinvPair;
+  const double* inva = base;
+  const double* invb = base + kPinvPanel;
+  const double* X = base + 2 * kPinvPanel;
   const int* ptype = P.ptype + P.ioff[p];
-  // a 2x2 pivot split by the boundary: its coupling slot L[b1, b1 - 1] is
-  // not part of L and must not enter the update of row b1
+  // a 2x2 pivot split by the step boundary: its coupling slot L[b1, b1 - 1]
+  // is not part of L and must not enter the update of row b1
   const bool cut = b1 < npiv && ptype[b1 - 1] == 2;
-  // right-hand-side independent operands: L[i, panel] and B^{-1}[lane, panel],
-  // loaded once from clamped (always valid) addresses
+  const int i = i0 + lane;
   const size_t irow = (size_t)min(i, m - 1);
-  double lrow[CQ], ainv[CQ];
+  // right-hand-side independent operands, loaded once from clamped (always
+  // valid) addresses
+  double lrow[CR], aa[CQ], ax[CQ], ab[CQ];
+#pragma unroll
+  for (int q = 0; q < CR; q++) {
+    const int c = wv * CR + q;
+    const double lv = (double)L[irow + (size_t)(b0 + min(c, w - 1)) * m];
+    lrow[q] = (i < m && c < w && !(cut && i == b1 && c == w - 1)) ? lv : 0.0;
+  }
 #pragma unroll
   for (int q = 0; q < CQ; q++) {
     const int c = wv * CQ + q;
-    const double lv = (double)L[irow + (size_t)(b0 + min(c, w - 1)) * m];
-    lrow[q] = (i < m && c < w && !(cut && i == b1 && c == w - 1)) ? lv : 0.0;
-    ainv[q] = c < w ? inv[lane + c * kPB] : 0.0;
+    aa[q] = c < wa ? inva[lane + c * kPB] : 0.0;
+    ax[q] = (c < wa && wb > 0) ? X[lane + c * kPB] : 0.0;
+    ab[q] = c < wb ? invb[lane + c * kPB] : 0.0;
   }
   const int* vars = P.vars + P.ioff[p];
   const double* dv = P.dv + P.ioff[p];
@@ -1963,26 +2030,36 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   {
     const int r = blockIdx.y;  // one right-hand side per grid row
     double* y = Y + (long long)r * ystride + P.ioff[p];
-    double a = 0.0;
+    double sa = 0.0, sb = 0.0;
 #pragma unroll
     for (int q = 0; q < CQ; q++) {
       const int c = wv * CQ + q;
-      if (c < w) a += ainv[q] * y[b0 + c];
+      const double ya = c < wa ? y[b0 + c] : 0.0;
+      const double yb = c < wb ? y[b0 + kPB + c] : 0.0;
+      sa += aa[q] * ya;
+      sb += ax[q] * ya + ab[q] * yb;
     }
-    part[wv][lane] = a;
+    part[0][wv][lane] = sa;
+    part[1][wv][lane] = sb;
     __syncthreads();
-    if (wv == 0)
-      xp[lane] = lane < w ? ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]
-                          : 0.0;
+    if (wv == 0) {
+      xp[lane] = lane < wa ? ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) +
+                                 part[0][3][lane]
+                           : 0.0;
+      xp[kPB + lane] = lane < wb ? ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) +
+                                       part[1][3][lane]
+                                 : 0.0;
+    }
     __syncthreads();
     double g = 0.0;
 #pragma unroll
-    for (int q = 0; q < CQ; q++) g += lrow[q] * xp[wv * CQ + q];
-    part[wv][lane] = g;
+    for (int q = 0; q < CR; q++) g += lrow[q] * xp[wv * CR + q];
+    part[0][wv][lane] = g;
     __syncthreads();
     double* xr = x + (size_t)r * ldx;
     if (wv == 0 && i < m) {
-      const double sum = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+      const double sum =
+          ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
       if (i < npiv) {
         const double yn = y[i] - sum;
         y[i] = yn;
@@ -1998,93 +2075,118 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
         P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)] -= sum;
       }
     }
-    if (tile == 0 && wv == 1 && lane < w) {
-      // 1x1 pivots and 2x2 pivots inside the panel; the split pairs are
-      // written above (second pivot first in the panel: by the step before)
-      const int j = b0 + lane;
-      const int ty = ptype[j];
-      if (ty == 1) {
-        xr[unpiv(vars[j])] = xp[lane] / dv[j];
-      } else if (ty == 2 && lane + 1 < w) {
-        const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
-        const double dinv = 1.0 / (d11 * d22 - d21 * d21);
-        const double ta = xp[lane], tb = xp[lane + 1];
-        xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
-        xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
+    if (tile == 0 && (wv == 1 || wv == 2)) {
+      // 1x1 pivots and 2x2 pivots inside the step (also across its 64-column
+      // blocks); a pair split by the step boundary is written above (second
+      // pivot first in the step: by the step before)
+      const int jj = (wv - 1) * kPB + lane;
+      if (jj < w) {
+        const int j = b0 + jj;
+        const int ty = ptype[j];
+        if (ty == 1) {
+          xr[unpiv(vars[j])] = xp[jj] / dv[j];
+        } else if (ty == 2 && jj + 1 < w) {
+          const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
+          const double dinv = 1.0 / (d11 * d22 - d21 * d21);
+          const double ta = xp[jj], tb = xp[jj + 1];
+          xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
+          xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
+        }
       }
     }
     __syncthreads();
   }
 }
 
-// Backward panel step s: panel q = (number of panels) - 1 - s, from the
-// bottom.  Y holds the right-hand side of L11^T (k_bwd_gemv), updated by the
-// earlier steps.  Workgroup (node, tile):
-//   x_q = B_q^{-T} y[b0, b1)                         (lanes = rows of x_q)
-//   y[c] -= L[b0:b1, c]^T x_q  for its kStepCols columns c < b0
-// (one wave per column, lanes over the panel rows).  Tile 0 writes x_q.
+// Backward step s: step q = (number of steps) - 1 - s, from the bottom.  Y
+// holds the right-hand side of L11^T (k_bwd_gemv), updated by the earlier
+// steps.  Workgroup (node, tile):
+//   x_b = B_b^{-T} y_b,  x_a = B_a^{-T} y_a + X^T y_b        (lanes = rows)
+//   y[c] -= L[b0:b1, c]^T x  for its kStepCols columns c < b0
+// (one wave per column, lanes over the step's rows, two per lane).  Tile 0
+// writes x.
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
                                                    const int* wtile, int s, double* x,
                                                    int ldx, int nrhs, double* Y,
                                                    long long ystride) {
   constexpr int NW = kTPB / kWave;
-  constexpr int CQ = kPB / NW;           // panel rows per wave in x_q = B^{-T} y
-  constexpr int CPW = kStepCols / NW;    // tile columns per wave
-  __shared__ double xp[kPB];
-  __shared__ double part[NW][kWave];
+  constexpr int CQ = kPB / NW;         // block rows per wave (inverse products)
+  constexpr int CPW = kStepCols / NW;  // tile columns per wave
+  __shared__ double xp[kPS];
+  __shared__ double part[2][NW][kWave];
   const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
   const int m = P.m[p], npiv = P.npiv[p];
-  const int q = (npiv + kPB - 1) / kPB - 1 - s;
+  const int q = (npiv + kPS - 1) / kPS - 1 - s;
   if (q < 0) return;
-  const int b0 = q * kPB, w = min(kPB, npiv - b0);
+  const int b0 = q * kPS, w = min(kPS, npiv - b0);
+  const int wa = min(w, kPB), wb = w - wa;
   const int c0 = tile * kStepCols;
   if (tile > 0 && c0 >= b0) return;
   const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
   const T* L = front_ptr<T>(P, p);
-  const double* invt = P.pinv + P.pinvoff[p] + (long long)q * kPinvPanel + kPB * kPB;
-  const size_t prow = (size_t)(b0 + min(lane, w - 1));
+  const double* base = P.pinv + P.pinvoff[p] + (long long)q * kPinvPair;
+  const double* invaT = base + kPB * kPB;
+  const double* invbT = base + kPinvPanel + kPB * kPB;
+  const double* XT = base + 2 * kPinvPanel + kPB * kPB;
   const int cmax = max(b0 - 1, 0);
-  // a 2x2 pivot split by the boundary: L[b0, b0 - 1] is not part of L
+  // a 2x2 pivot split by the step boundary: L[b0, b0 - 1] is not part of L
   const bool cut = b0 > 0 && P.ptype[P.ioff[p] + b0 - 1] == 2;
-  double lcol[CPW], ainv[CQ];
+  double lcol[CPW][2], ta[CQ], tx[CQ], tb[CQ];
 #pragma unroll
   for (int k = 0; k < CPW; k++) {
     const int c = c0 + wv * CPW + k;
-    const double lv = (double)L[prow + (size_t)min(c, cmax) * m];
-    lcol[k] = (lane < w && c < b0 && !(cut && lane == 0 && c == b0 - 1)) ? lv : 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int rr = h * kPB + lane;
+      const double lv = (double)L[(size_t)(b0 + min(rr, w - 1)) + (size_t)min(c, cmax) * m];
+      lcol[k][h] = (rr < w && c < b0 && !(cut && rr == 0 && c == b0 - 1)) ? lv : 0.0;
+    }
   }
 #pragma unroll
   for (int k = 0; k < CQ; k++) {
     const int i = wv * CQ + k;
-    ainv[k] = i < w ? invt[lane + i * kPB] : 0.0;
+    ta[k] = i < wa ? invaT[lane + i * kPB] : 0.0;
+    tx[k] = i < wb ? XT[lane + i * kPB] : 0.0;
+    tb[k] = i < wb ? invbT[lane + i * kPB] : 0.0;
   }
   const int* vars = P.vars + P.ioff[p];
   {
     const int r = blockIdx.y;  // one right-hand side per grid row
     double* y = Y + (long long)r * ystride + P.ioff[p];
-    double a = 0.0;
+    double sa = 0.0, sb = 0.0;
 #pragma unroll
     for (int k = 0; k < CQ; k++) {
       const int i = wv * CQ + k;
-      if (i < w) a += ainv[k] * y[b0 + i];
+      const double ya = i < wa ? y[b0 + i] : 0.0;
+      const double yb = i < wb ? y[b0 + kPB + i] : 0.0;
+      sa += ta[k] * ya + tx[k] * yb;
+      sb += tb[k] * yb;
     }
-    part[wv][lane] = a;
+    part[0][wv][lane] = sa;
+    part[1][wv][lane] = sb;
     __syncthreads();
-    if (wv == 0)
-      xp[lane] = lane < w ? ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]
-                          : 0.0;
+    if (wv == 0) {
+      xp[lane] = lane < wa ? ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) +
+                                 part[0][3][lane]
+                           : 0.0;
+      xp[kPB + lane] = lane < wb ? ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) +
+                                       part[1][3][lane]
+                                 : 0.0;
+    }
     __syncthreads();
-    const double xl = xp[lane];
+    const double xl0 = xp[lane], xl1 = xp[kPB + lane];
 #pragma unroll
     for (int k = 0; k < CPW; k++) {
-      double g = lcol[k] * xl;
+      double g = lcol[k][0] * xl0 + lcol[k][1] * xl1;
       for (int off = kWave / 2; off > 0; off >>= 1) g += __shfl_xor(g, off);
       const int c = c0 + wv * CPW + k;
       if (lane == 0 && c < b0) y[c] -= g;
     }
-    if (tile == 0 && wv == 1 && lane < w)
-      x[(size_t)r * ldx + unpiv(vars[b0 + lane])] = xp[lane];
+    if (tile == 0 && (wv == 1 || wv == 2)) {
+      const int jj = (wv - 1) * kPB + lane;
+      if (jj < w) x[(size_t)r * ldx + unpiv(vars[b0 + jj])] = xp[jj];
+    }
     __syncthreads();
   }
 }
@@ -2276,11 +2378,11 @@ struct Handle {
   DBuf<double> d_Y;
   // panel inverses of the large nodes: one (node, panel) item per block
   std::vector<long long> pinvoff;
-  std::vector<int> inv_node, inv_panel;
+  std::vector<int> inv_node, inv_panel, inv2_node, inv2_pair;
   long long pinv_total = 0;
   std::vector<int> lvl_steps;  // per level: panel steps needed by the last factorization
   DBuf<long long> d_pinvoff;
-  DBuf<int> d_inv_node, d_inv_panel;
+  DBuf<int> d_inv_node, d_inv_panel, d_inv2_node, d_inv2_pair;
   DBuf<double> d_pinv;
   int y_nrhs = 0;
   DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_norm, d_dv, d_dv2;
@@ -2445,9 +2547,13 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   H.pinvoff.assign(N, 0);
   H.inv_node.clear();
   H.inv_panel.clear();
+  H.inv2_node.clear();
+  H.inv2_pair.clear();
   long long pv = 0;
-  // panels of a large node: its pivots never exceed cap - cs
+  // panels of a large node: its pivots never exceed cap - cs; a solve step
+  // covers two panels
   auto npanels = [&](int k) { return (H.cap[k] - H.cs[k] + kPB - 1) / kPB; };
+  auto nsteps = [&](int k) { return (npanels(k) + 1) / 2; };
   for (int l = 0; l < nlev; l++) {
     for (int q = H.lds_ptr[l]; q < H.lds_ptr[l + 1]; q++) H.sv_nodes.push_back(H.lds_nodes[q]);
     for (int q = H.small_ptr[l]; q < H.small_ptr[l + 1]; q++)
@@ -2460,12 +2566,16 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
         H.bg_tile.push_back(t);
       }
       H.pinvoff[k] = pv;
-      pv += npanels(k) * kPinvPanel;
+      pv += nsteps(k) * kPinvPair;
       for (int t = 0; t < npanels(k); t++) {
         H.inv_node.push_back(k);
         H.inv_panel.push_back(t);
       }
-      maxp = std::max(maxp, npanels(k));
+      for (int t = 0; 2 * t + 1 < npanels(k); t++) {
+        H.inv2_node.push_back(k);
+        H.inv2_pair.push_back(t);
+      }
+      maxp = std::max(maxp, nsteps(k));
     }
     H.sv_ptr[l + 1] = (int)H.sv_nodes.size();
     H.bg_ptr[l + 1] = (int)H.bg_node.size();
@@ -2480,13 +2590,13 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
       H.sv_lt[g] = std::max(H.sv_lt[g], (H.cap[k] + 7) & ~7);
       H.sv_lb[g] = std::max(H.sv_lb[g], std::min(kSB, H.cap[k] - H.cs[k]));
     }
-    // forward step t: rows [b1, m) with b1 > t*kPB; tile 0 always exists
+    // forward step t: rows [b1, m) with b1 > t*kPS; tile 0 always exists
     H.fs_lvl[l] = (int)H.fs_ptr.size() - 1;
     for (int t = 0; t < maxp; t++) {
       for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
         const int k = H.large_nodes[q];
-        if (t >= npanels(k)) continue;
-        const int nt = std::max(1, (H.cap[k] - t * kPB + kStepRows - 1) / kStepRows);
+        if (t >= nsteps(k)) continue;
+        const int nt = std::max(1, (H.cap[k] - t * kPS + kStepRows - 1) / kStepRows);
         for (int u = 0; u < nt; u++) {
           H.fs_node.push_back(k);
           H.fs_tile.push_back(u);
@@ -2494,13 +2604,13 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
       }
       H.fs_ptr.push_back((int)H.fs_node.size());
     }
-    // backward step t: panel (panels - 1 - t), columns before it
+    // backward step t: step (steps - 1 - t), columns before it
     H.bs_lvl[l] = (int)H.bs_ptr.size() - 1;
     for (int t = 0; t < maxp; t++) {
       for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
         const int k = H.large_nodes[q];
-        if (t >= npanels(k)) continue;
-        const int cols = (npanels(k) - 1 - t) * kPB;
+        if (t >= nsteps(k)) continue;
+        const int cols = (nsteps(k) - 1 - t) * kPS;
         const int nt = std::max(1, (cols + kStepCols - 1) / kStepCols);
         for (int u = 0; u < nt; u++) {
           H.bs_node.push_back(k);
@@ -2629,6 +2739,8 @@ static void upload_static(Handle& H) {
   H.d_pinvoff.upload(H.pinvoff);
   H.d_inv_node.upload(H.inv_node);
   H.d_inv_panel.upload(H.inv_panel);
+  H.d_inv2_node.upload(H.inv2_node);
+  H.d_inv2_pair.upload(H.inv2_pair);
   H.d_pinv.alloc((size_t)H.pinv_total);
   H.y_nrhs = 0;
   H.d_F.alloc(((size_t)H.f_total * H.esize + 7) / 8);
@@ -2860,6 +2972,9 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
     ProfScope ps_(H, st, KC_INVERT);
     hipLaunchKernelGGL(k_panel_inv<T>, dim3((unsigned)H.inv_node.size()), dim3(kWave), 0, st, P,
                        H.d_inv_node.p, H.d_inv_panel.p);
+    if (!H.inv2_node.empty())
+      hipLaunchKernelGGL(k_panel_inv2<T>, dim3((unsigned)H.inv2_node.size()), dim3(kTPB), 0, st,
+                         P, H.d_inv2_node.p, H.d_inv2_pair.p);
   }
   HIPCHECK(hipGetLastError());
 }
@@ -2949,7 +3064,7 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
   H.lvl_steps.assign(nlev, 0);
   for (int l = 0; l < nlev; l++)
     for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++)
-      H.lvl_steps[l] = std::max(H.lvl_steps[l], (H.h_npiv[H.large_nodes[q]] + kPB - 1) / kPB);
+      H.lvl_steps[l] = std::max(H.lvl_steps[l], (H.h_npiv[H.large_nodes[q]] + kPS - 1) / kPS);
   int nd = 0, n2 = 0;
   long long stored = 0;
   for (int k = 0; k < H.N; k++) {
