@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -126,13 +127,15 @@ struct DevPlan {
   double* W;
   double* U;
   // large nodes: inverses of the kPB x kPB diagonal blocks of L11, two
-  // copies per block (column major, then transposed), at pinvoff[node]
+  // copies per block (column major, then transposed), and per pair of
+  // blocks the coupling term of the composed inverse, at pinvoff[node]
   double* pinv;
   const long long* pinvoff;
   // numeric input
   const double* vals;
   const double* diag;
   double ustab, ptol;
+  int spec;            // large-front panels choose their pivots speculatively
 };
 
 __device__ inline int unpiv(int v) { return v >= 0 ? v : -v - 1; }
@@ -979,6 +982,25 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
 // --------------------------------------------------------------------------
 constexpr int kWCRows = kNBLarge + 2;
 
+// Pivot arithmetic shared by the sequential and the speculative panel paths,
+// evaluated without contraction so that both make the same decisions.
+__device__ inline double piv_det(double d11, double d22, double d21) {
+#pragma clang fp contract(off)
+  return d11 * d22 - d21 * d21;
+}
+__device__ inline double piv_delta(double det, double awkk, double awii, double awik) {
+#pragma clang fp contract(off)
+  return det / fmax(awii + awik, awkk + awik);
+}
+__device__ inline void piv_inv2(double d11, double d21, double d22, double& i00, double& i01,
+                                double& i11) {
+#pragma clang fp contract(off)
+  const double detinv = 1.0 / piv_det(d11, d22, d21);
+  i00 = d22 * detinv;
+  i01 = -d21 * detinv;
+  i11 = d11 * detinv;
+}
+
 template <int NW>
 struct RRShared {
   double pb[2][NW];  // column reduction: best candidate |v|
@@ -992,6 +1014,13 @@ struct RRShared {
   double swii, swik;
   int cT[kNBLarge];  // per panel column: 1 1x1, 2/3 first/second of 2x2, 4 unscaled
   int npos, nneg, n2;
+  // speculative panel: the steps chosen from the leading rows
+  int sp_n, sp_end;
+  unsigned sp_fail;
+  int sp_c[kNBLarge], sp_t[kNBLarge];
+  double sp_thr[kNBLarge], sp_awik[kNBLarge];
+  double sp_m[kNBLarge][3];  // 1/d, or the 2x2 inverse i00, i01, i11
+  double sp_d[kNBLarge][3];  // d, or d11, d21, d22
 };
 
 // Workgroup barrier that orders LDS only: it waits for this wave's LDS
@@ -1079,38 +1108,40 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
   constexpr int RS = R * NT;        // rows of a panel column in LDS
   extern __shared__ double s_rr[];  // Ls[NB][RS] of T, then vars[m], pre[m]
   __shared__ RRShared<NW> sh;
-  __shared__ PanelState s_st;
   const int p = nodes[blockIdx.x];
-  if (P.stat[p]) return;
+  // everything per node depends on p alone: one round of loads
+  const int stat = P.stat[p];
+  const PanelState st0 = P.ps[p];
+  const int m = P.m[p];
+  const int mode = P.mode[p];
+  const long long io = P.ioff[p];
+  T* F = front_ptr<T>(P, p);
+  double* Wg = P.W + P.woff[p];
+  const int ldw = P.cap[p];
+  if (stat) return;
   const int tid = threadIdx.x;
   const int wv = tid / kWave, lane = tid % kWave;
   if (tid == 0) {
-    s_st = P.ps[p];
-    sh.npos = s_st.npos;
-    sh.nneg = s_st.nneg;
-    sh.n2 = s_st.n2;
+    sh.npos = st0.npos;
+    sh.nneg = st0.nneg;
+    sh.n2 = st0.n2;
   }
-  __syncthreads();
-  int k = s_st.k, nc = s_st.nc, kw = 0;
+  int k = st0.k, nc = st0.nc, kw = 0;
   const int kstart = k;
   if (k < nc) {
-    const int m = P.m[p];
-    const int mode = P.mode[p];
     const int nb = max(2, min(NB, m));
-    const long long io = P.ioff[p];
-    T* F = front_ptr<T>(P, p);
     const int ld = m;
-    double* Wg = P.W + P.woff[p];
-    const int ldw = P.cap[p];
     const double ustab = P.ustab, ptol = P.ptol;
     const double alpha = 0.6403882032022076;  // (1 + sqrt(17)) / 8
     T* Ls = (T*)s_rr;
     int* vars = (int*)(Ls + (size_t)NB * RS);
     int* pre = vars + m;
-    for (int i = k + tid; i < m; i += NT) {
-      vars[i] = P.vars[io + i];
-      pre[i] = P.pre[io + i];
-    }
+    auto ld_index = [&]() {
+      for (int i = k + tid; i < m; i += NT) {
+        vars[i] = P.vars[io + i];
+        pre[i] = P.pre[io + i];
+      }
+    };
     auto rowof = [&](int j) { return kstart + tid + j * NT; };
     auto lsx = [&](int c, int j) -> T& { return Ls[(size_t)c * RS + j * NT + tid]; };
     auto lsr = [&](int c, int r) -> T& { return Ls[(size_t)c * RS + (r - kstart)]; };
@@ -1129,7 +1160,271 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
       ld_col(k + 2, f2);
       ld_col(k + 3, f3);
     };
-    reload();
+    // Speculative panel (threshold pivoting).  The pivots are chosen by wave
+    // 0 from its rows alone (positions kstart .. kstart + 63, one per lane):
+    // a 1x1 pivot, or a 2x2 pivot with the next position, whenever the tests
+    // of the sequential loop below pass against the three rows after the
+    // diagonal.  Every row then forms its W and L entries for those choices
+    // (the same FMA chains as the sequential loop, so the same bits) and
+    // checks them: the choice is the sequential loop's exactly when no row
+    // exceeds the bound it implies (|W| <= |d| / ustab for a 1x1 pivot; the
+    // pivot row's entry the largest candidate and delta / ustab the bound for
+    // a 2x2 pivot; a NaN fails).  Both bounds only grow with more rows, so a
+    // test that fails on the leading rows fails on the front.  The first
+    // choice that some row contradicts, and everything after it, is left to
+    // the sequential loop.  One barrier for the choices and one for the
+    // checks replace the per-pivot reductions and barriers.
+    const bool spec = P.spec && mode == 0;
+    if (spec) {
+      T fin[NB][R];  // front columns kstart .. kstart + NB - 1 of the owned rows
+#pragma unroll
+      for (int c = 0; c < NB; c++) {
+        const size_t co = (size_t)min(kstart + c, m - 1) * ld;
+#pragma unroll
+        for (int j = 0; j < R; j++) fin[c][j] = F[min(rowof(j), m - 1) + co];
+      }
+      ld_index();
+      if (tid == 0) sh.sp_fail = 0u;
+      if (wv == 0) {
+        const int r = kstart + lane;
+        const bool rin = r < m;
+        double xs[NB];  // W columns of the choices made so far, at row r
+#pragma unroll
+        for (int c = 0; c < NB; c++) xs[c] = 0.0;
+        int ns = 0, next = 0;
+        bool go = true;
+#pragma unroll
+        for (int cc = 0; cc < NB - 1; cc++) {
+          if (!go || cc != next) continue;
+          const int kp = kstart + cc;
+          if (cc >= nb - 1 || kp >= nc) {
+            go = false;
+            continue;
+          }
+          double x = (double)fin[cc][0];
+#pragma unroll
+          for (int c = 0; c < cc; c++) x = __builtin_fma(-(double)fin[c][0], lane_d(xs[c], cc), x);
+          const double wkk = lane_d(x, cc), awkk = fabs(wkk);
+          double lcol = 0.0;
+#pragma unroll
+          for (int t = 1; t <= 3; t++)
+            if (kp + t < m) lcol = fmax(lcol, fabs(lane_d(x, cc + t)));
+          int type = 0;
+          double thr = 0.0, awik = 0.0, wik = 0.0, wii = 0.0, y = 0.0;
+          if (awkk >= ptol && awkk >= ustab * lcol) {
+            type = 1;
+            thr = awkk;
+          } else if (kp + 1 < nc) {
+            wik = lane_d(x, cc + 1);
+            awik = fabs(wik);
+            bool adj = awik >= 0.0;  // false for NaN
+#pragma unroll
+            for (int t = 2; t <= 3; t++)
+              if (kp + t < nc && fabs(lane_d(x, cc + t)) > awik) adj = false;
+            if (adj) {
+              // column k+1 updated; its row-k entry is F[k+1, k] with the
+              // update chain of row k (as in the sequential loop)
+              const double f1 = lane_d((double)fin[cc][0], cc + 1);
+              double yv = lane == cc ? f1 : (double)fin[cc + 1][0];
+#pragma unroll
+              for (int c = 0; c < cc; c++)
+                yv = __builtin_fma(-(double)fin[c][0], lane_d(xs[c], cc + 1), yv);
+              wii = lane_d(yv, cc + 1);
+              double lrow = fabs(lane_d(yv, cc));
+#pragma unroll
+              for (int t = 2; t <= 3; t++)
+                if (kp + t < m) lrow = fmax(lrow, fabs(lane_d(yv, cc + t)));
+              const double awii = fabs(wii);
+              if (!(awii >= ptol && awii >= ustab * lrow)) {
+                const double det = fabs(piv_det(wkk, wii, wik));
+                if (det >= ptol) {
+                  const double delta = piv_delta(det, awkk, awii, awik);
+                  if (delta >= ustab * lcol && delta >= ustab * lrow) {
+                    type = 2;
+                    thr = delta;
+                    y = yv;
+                  }
+                }
+              }
+            }
+          }
+          // this wave's rows against the bounds of the choice
+          bool bad = false;
+          if (rin && type == 1 && r > kp) bad = !(ustab * fabs(x) <= thr);
+          if (rin && type == 2 && r > kp) {
+            bad = !(ustab * fabs(x) <= thr);
+            if (r > kp + 1) {
+              bad = bad || !(ustab * fabs(y) <= thr);
+              if (r < nc) bad = bad || !(fabs(x) <= awik);
+            }
+          }
+          if (type == 0 || __ballot(bad) != 0ull) {
+            go = false;
+            continue;
+          }
+          if (type == 1) {
+            const double dinv = wkk == 0.0 ? 0.0 : 1.0 / wkk;
+            if (rin && r >= kp) {
+              const T l = (T)(x * dinv);
+              fin[cc][0] = l;
+              lsx(cc, 0) = l;
+              Wg[r + (size_t)cc * ldw] = x;
+              if (lane < kWCRows) sh.wc[lane][cc] = x;
+            }
+            if (lane == 0) {
+              sh.sp_m[ns][0] = dinv;
+              sh.sp_d[ns][0] = wkk;
+            }
+          } else {
+            double i00, i01, i11;
+            piv_inv2(wkk, wik, wii, i00, i01, i11);
+            if (rin && r >= kp) {
+              const T l0 = (T)__builtin_fma(x, i00, y * i01);
+              const T l1 = (T)__builtin_fma(x, i01, y * i11);
+              fin[cc][0] = l0;
+              fin[cc + 1][0] = l1;
+              lsx(cc, 0) = l0;
+              lsx(cc + 1, 0) = l1;
+              Wg[r + (size_t)cc * ldw] = x;
+              Wg[r + (size_t)(cc + 1) * ldw] = y;
+              if (lane < kWCRows) {
+                sh.wc[lane][cc] = x;
+                sh.wc[lane][cc + 1] = y;
+              }
+            }
+            if (lane == 0) {
+              sh.sp_m[ns][0] = i00;
+              sh.sp_m[ns][1] = i01;
+              sh.sp_m[ns][2] = i11;
+              sh.sp_d[ns][0] = wkk;
+              sh.sp_d[ns][1] = wik;
+              sh.sp_d[ns][2] = wii;
+            }
+            xs[cc + 1] = y;
+          }
+          xs[cc] = x;
+          if (lane == 0) {
+            sh.sp_c[ns] = cc;
+            sh.sp_t[ns] = type;
+            sh.sp_thr[ns] = thr;
+            sh.sp_awik[ns] = awik;
+          }
+          ns++;
+          next = cc + type;
+        }
+        if (lane == 0) {
+          sh.sp_n = ns;
+          sh.sp_end = next;
+        }
+      }
+      lds_sync();
+      // the other rows (all below position kstart + 63, so below every
+      // pivot of the panel): W and L entries of the choices, and the checks
+      const int nsp = sh.sp_n;
+      unsigned fail = 0u;
+      {
+        int s = 0, next = 0;
+#pragma unroll
+        for (int cc = 0; cc < NB - 1; cc++) {
+          if (s >= nsp || cc != next) continue;
+          const int type = sh.sp_t[s];
+          const double thr = sh.sp_thr[s];
+          double w0[NB], w1[NB];
+#pragma unroll
+          for (int c = 0; c < cc; c++) {
+            w0[c] = sh.wc[cc][c];
+            w1[c] = sh.wc[cc + 1][c];
+          }
+          if (type == 1) {
+            const double dinv = sh.sp_m[s][0];
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+              const int r = rowof(j);
+              if ((wv == 0 && j == 0) || r >= m) continue;
+              double x = (double)fin[cc][j];
+#pragma unroll
+              for (int c = 0; c < cc; c++) x = __builtin_fma(-(double)fin[c][j], w0[c], x);
+              const T l = (T)(x * dinv);
+              fin[cc][j] = l;
+              lsx(cc, j) = l;
+              Wg[r + (size_t)cc * ldw] = x;
+              if (!(ustab * fabs(x) <= thr)) fail |= 1u << s;
+            }
+          } else {
+            const double awik = sh.sp_awik[s];
+            const double i00 = sh.sp_m[s][0], i01 = sh.sp_m[s][1], i11 = sh.sp_m[s][2];
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+              const int r = rowof(j);
+              if ((wv == 0 && j == 0) || r >= m) continue;
+              double x = (double)fin[cc][j], y = (double)fin[cc + 1][j];
+#pragma unroll
+              for (int c = 0; c < cc; c++) {
+                x = __builtin_fma(-(double)fin[c][j], w0[c], x);
+                y = __builtin_fma(-(double)fin[c][j], w1[c], y);
+              }
+              const T l0 = (T)__builtin_fma(x, i00, y * i01);
+              const T l1 = (T)__builtin_fma(x, i01, y * i11);
+              fin[cc][j] = l0;
+              fin[cc + 1][j] = l1;
+              lsx(cc, j) = l0;
+              lsx(cc + 1, j) = l1;
+              Wg[r + (size_t)cc * ldw] = x;
+              Wg[r + (size_t)(cc + 1) * ldw] = y;
+              if (!(ustab * fabs(x) <= thr) || !(ustab * fabs(y) <= thr) ||
+                  (r < nc && !(fabs(x) <= awik)))
+                fail |= 1u << s;
+            }
+          }
+          s++;
+          next = cc + type;
+        }
+      }
+      if (fail) atomicOr(&sh.sp_fail, fail);
+      lds_sync();
+      const unsigned fm = sh.sp_fail;
+      const int nok = fm ? min(nsp, (int)__builtin_ctz(fm)) : nsp;
+      kw = nok < nsp ? sh.sp_c[nok] : sh.sp_end;
+      k = kstart + kw;
+      // the accepted steps' D entries, pivot types and inertia, one step per
+      // thread (the counts are sums)
+      if (tid < nok) {
+        const int s = tid;
+        const int c = sh.sp_c[s], kp = kstart + c;
+        if (sh.sp_t[s] == 1) {
+          const double d = sh.sp_d[s][0];
+          F[kp + (size_t)kp * ld] = (T)d;
+          P.ptype[io + kp] = 1;
+          P.dv[io + kp] = d;
+          atomicAdd(d >= 0.0 ? &sh.npos : &sh.nneg, 1);
+          sh.cT[c] = 1;
+        } else {
+          const double d11 = sh.sp_d[s][0], d21 = sh.sp_d[s][1], d22 = sh.sp_d[s][2];
+          F[kp + (size_t)kp * ld] = (T)d11;
+          F[kp + 1 + (size_t)kp * ld] = (T)d21;
+          F[kp + 1 + (size_t)(kp + 1) * ld] = (T)d22;
+          P.dv[io + kp] = d11;
+          P.dv2[io + kp] = d21;
+          P.dv[io + kp + 1] = d22;
+          vars[kp] = -(vars[kp] + 1);
+          vars[kp + 1] = -(vars[kp + 1] + 1);
+          P.ptype[io + kp] = 2;
+          P.ptype[io + kp + 1] = 3;
+          if (piv_det(d11, d22, d21) >= 0.0) atomicAdd(d11 >= 0.0 ? &sh.npos : &sh.nneg, 2);
+          else {
+            atomicAdd(&sh.npos, 1);
+            atomicAdd(&sh.nneg, 1);
+          }
+          atomicAdd(&sh.n2, 1);
+          sh.cT[c] = 2;
+          sh.cT[c + 1] = 3;
+        }
+      }
+    } else {
+      ld_index();
+    }
+    // the sequential loop, for what the speculation left
+    if (kw < nb - 1 && k < nc) reload();
     __syncthreads();
     int it = 0;
     // Look-ahead: each step also forms the updates of columns k+1 and k+2 by
@@ -1330,9 +1625,9 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
             step = 1;
             kpiv = imax;
           } else {
-            const double det = fabs(wkk * wii - wik * wik);
+            const double det = fabs(piv_det(wkk, wii, wik));
             if (det >= ptol) {
-              const double delta = det / fmax(awii + awik, awkk + awik);
+              const double delta = piv_delta(det, awkk, awii, awik);
               if (delta >= ustab * colmax && delta >= ustab * rowmax) {
                 step = 2;
                 kpiv = imax;
@@ -1427,9 +1722,9 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
         }
       } else if (step == 2) {
         const double d11 = wkk, d21 = wik, d22 = wii;
-        const double det = d11 * d22 - d21 * d21;
-        const double detinv = 1.0 / det;
-        const double i00 = d22 * detinv, i01 = -d21 * detinv, i11 = d11 * detinv;
+        const double det = piv_det(d11, d22, d21);
+        double i00, i01, i11;
+        piv_inv2(d11, d21, d22, i00, i01, i11);
 #pragma unroll
         for (int j = 0; j < R; j++) {
           const int r = rowof(j);
@@ -1507,7 +1802,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
       }
       it++;
     }
-    __syncthreads();
+    lds_sync();
     // L of the panel below each pivot block (the D entries were written at
     // the pivot)
     for (int c = 0; c < kw; c++) {
@@ -1533,7 +1828,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
     }
   }
   if (tid == 0) {
-    PanelState st = s_st;
+    PanelState st = st0;
     st.k = k;
     st.nc = nc;
     st.kstart = kstart;
@@ -2331,6 +2626,10 @@ enum KernelClass {
 
 struct Handle {
   kkt_ldl_options opt;
+  // large-front panels choose pivots speculatively and verify them; the
+  // environment variable KKT_LDL_PANEL_SPECULATION=0 selects the sequential
+  // panel everywhere (same factors, bit for bit)
+  int panel_spec = 1;
   // optional per-kernel-class device timing (events around every launch)
   bool profiling = false;
   std::vector<hipEvent_t> prof_ev;   // pairs
@@ -2798,6 +3097,7 @@ static DevPlan dev_plan(Handle& H) {
   P.diag = H.has_diag ? H.d_diag.p : nullptr;
   P.ustab = H.opt.ustab;
   P.ptol = H.opt.pivot_tol;
+  P.spec = H.panel_spec;
   return P;
 }
 
@@ -3356,6 +3656,7 @@ int kkt_ldl_create(int n, const int* rowp, const int* cols,
     if (!out || !rowp || !cols) throw Error(-1, "null argument");
     std::unique_ptr<kkt_ldl_handle> h(new kkt_ldl_handle());
     kktldl::create_impl(h->h, n, rowp, cols, opts);
+    if (const char* e = std::getenv("KKT_LDL_PANEL_SPECULATION")) h->h.panel_spec = std::atoi(e) != 0;
     *out = h.release();
     return 0;
   });
