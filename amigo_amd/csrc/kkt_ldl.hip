@@ -1174,7 +1174,8 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
     // choice that some row contradicts, and everything after it, is left to
     // the sequential loop.  One barrier for the choices and one for the
     // checks replace the per-pivot reductions and barriers.
-    const bool spec = P.spec && mode == 0;
+    const bool spec = P.spec && mode != 2;
+    const bool bk = mode == 1;  // Bunch-Kaufman tests (the root front)
     if (spec) {
       T fin[NB][R];  // front columns kstart .. kstart + NB - 1 of the owned rows
 #pragma unroll
@@ -1211,7 +1212,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
             if (kp + t < m) lcol = fmax(lcol, fabs(lane_d(x, cc + t)));
           int type = 0;
           double thr = 0.0, awik = 0.0, wik = 0.0, wii = 0.0, y = 0.0;
-          if (awkk >= ptol && awkk >= ustab * lcol) {
+          if (bk ? (awkk > 0.0 && awkk >= alpha * lcol) : (awkk >= ptol && awkk >= ustab * lcol)) {
             type = 1;
             thr = awkk;
           } else if (kp + 1 < nc) {
@@ -1235,7 +1236,21 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
               for (int t = 2; t <= 3; t++)
                 if (kp + t < m) lrow = fmax(lrow, fabs(lane_d(yv, cc + t)));
               const double awii = fabs(wii);
-              if (!(awii >= ptol && awii >= ustab * lrow)) {
+              if (bk) {
+                // the second test passes with the leading rows' row maximum,
+                // so with the front's (no smaller): 1x1 at k; else a 2x2
+                // pivot when the third test fails (a 1x1 pivot at the partner
+                // or a zero column is left to the sequential loop)
+                if (awik > 0.0 && awkk == awkk) {
+                  if (awkk >= alpha * awik * (awik / lrow)) {
+                    type = 1;
+                    thr = -1.0;  // verified through the partner's maximality alone
+                  } else if (!(awii >= alpha * lrow)) {
+                    type = 2;
+                    y = yv;
+                  }
+                }
+              } else if (!(awii >= ptol && awii >= ustab * lrow)) {
                 const double det = fabs(piv_det(wkk, wii, wik));
                 if (det >= ptol) {
                   const double delta = piv_delta(det, awkk, awii, awik);
@@ -1250,8 +1265,15 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
           }
           // this wave's rows against the bounds of the choice
           bool bad = false;
-          if (rin && type == 1 && r > kp) bad = !(ustab * fabs(x) <= thr);
-          if (rin && type == 2 && r > kp) {
+          if (rin && type == 1 && r > kp)
+            bad = thr < 0.0 ? (r > kp + 1 && !(fabs(x) <= awik))
+                            : !((bk ? alpha : ustab) * fabs(x) <= thr);
+          if (rin && type == 2 && bk && r >= kp && r != kp + 1) {
+            // the second test must fail with this row's entry as the row maximum
+            bad = !(awkk < alpha * awik * (awik / fabs(y)));
+            if (r > kp + 1) bad = bad || !(fabs(x) <= awik);
+          }
+          if (rin && type == 2 && !bk && r > kp) {
             bad = !(ustab * fabs(x) <= thr);
             if (r > kp + 1) {
               bad = bad || !(ustab * fabs(y) <= thr);
@@ -1348,10 +1370,11 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
               fin[cc][j] = l;
               lsx(cc, j) = l;
               Wg[r + (size_t)cc * ldw] = x;
-              if (!(ustab * fabs(x) <= thr)) fail |= 1u << s;
+              const bool ok = thr < 0.0 ? fabs(x) <= sh.sp_awik[s] : (bk ? alpha : ustab) * fabs(x) <= thr;
+              if (!ok) fail |= 1u << s;
             }
           } else {
-            const double awik = sh.sp_awik[s];
+            const double awik = sh.sp_awik[s], awkk = fabs(sh.sp_d[s][0]);
             const double i00 = sh.sp_m[s][0], i01 = sh.sp_m[s][1], i11 = sh.sp_m[s][2];
 #pragma unroll
             for (int j = 0; j < R; j++) {
@@ -1371,9 +1394,11 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
               lsx(cc + 1, j) = l1;
               Wg[r + (size_t)cc * ldw] = x;
               Wg[r + (size_t)(cc + 1) * ldw] = y;
-              if (!(ustab * fabs(x) <= thr) || !(ustab * fabs(y) <= thr) ||
-                  (r < nc && !(fabs(x) <= awik)))
-                fail |= 1u << s;
+              const bool ok =
+                  bk ? (fabs(x) <= awik && awkk < alpha * awik * (awik / fabs(y)))
+                     : (ustab * fabs(x) <= thr && ustab * fabs(y) <= thr &&
+                        !(r < nc && !(fabs(x) <= awik)));
+              if (!ok) fail |= 1u << s;
             }
           }
           s++;
