@@ -1872,12 +1872,17 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
                                                  const int* wtile, int base) {
   typedef T vec4 __attribute__((ext_vector_type(4)));
   const int p = wnode[blockIdx.x];
-  if (P.stat[p]) return;
-  const PanelState st = P.ps[p];
-  const int kw = st.kw;
-  if (kw == 0) return;
-  const int m = P.m[p], k = st.k, kstart = st.kstart;
   const int t = wtile[blockIdx.x];
+  // everything per node depends on p alone: one round of loads
+  const int stat = P.stat[p];
+  const PanelState st = P.ps[p];
+  const int m = P.m[p];
+  T* F = front_ptr<T>(P, p);
+  const double* W = P.W + P.woff[p];
+  const int ldw = P.cap[p];
+  const int kw = st.kw;
+  if (stat || kw == 0) return;
+  const int k = st.k, kstart = st.kstart;
   int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while (bi * (bi + 1) / 2 > t) bi--;
   while ((bi + 1) * (bi + 2) / 2 <= t) bi++;
@@ -1887,9 +1892,6 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
   const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
   const int ri = i0 + 32 * (w >> 1), cj = j0 + 32 * (w & 1);
   if (ri + 32 <= cj) return;  // strictly above the diagonal
-  T* F = front_ptr<T>(P, p);
-  const double* W = P.W + P.woff[p];
-  const int ldw = P.cap[p];
   const int lr = l & 15, lk = l >> 4;
   constexpr bool f64 = sizeof(T) == 8;
   // output row of register r in this lane
@@ -2309,7 +2311,10 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   __shared__ double xp[kPS];
   __shared__ double part[2][NW][kWave];
   const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
+  // everything per node depends on p alone: one round of loads
   const int m = P.m[p], npiv = P.npiv[p];
+  const T* L = front_ptr<T>(P, p);
+  const long long pio = P.pinvoff[p], io = P.ioff[p];
   const int b0 = s * kPS;
   if (b0 >= npiv) return;
   const int b1 = min(npiv, b0 + kPS), w = b1 - b0;
@@ -2317,12 +2322,11 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   const int i0 = b1 + tile * kStepRows;
   if (tile > 0 && i0 >= m) return;
   const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
-  const T* L = front_ptr<T>(P, p);
-  const double* base = P.pinv + P.pinvoff[p] + (long long)s * kPinvPair;
+  const double* base = P.pinv + pio + (long long)s * kPinvPair;
   const double* inva = base;
   const double* invb = base + kPinvPanel;
   const double* X = base + 2 * kPinvPanel;
-  const int* ptype = P.ptype + P.ioff[p];
+  const int* ptype = P.ptype + io;
   // a 2x2 pivot split by the step boundary: its coupling slot L[b1, b1 - 1]
   // is not part of L and must not enter the update of row b1
   const bool cut = b1 < npiv && ptype[b1 - 1] == 2;
@@ -2344,12 +2348,12 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
     ax[q] = (c < wa && wb > 0) ? X[lane + c * kPB] : 0.0;
     ab[q] = c < wb ? invb[lane + c * kPB] : 0.0;
   }
-  const int* vars = P.vars + P.ioff[p];
-  const double* dv = P.dv + P.ioff[p];
-  const double* dv2 = P.dv2 + P.ioff[p];
+  const int* vars = P.vars + io;
+  const double* dv = P.dv + io;
+  const double* dv2 = P.dv2 + io;
   {
     const int r = blockIdx.y;  // one right-hand side per grid row
-    double* y = Y + (long long)r * ystride + P.ioff[p];
+    double* y = Y + (long long)r * ystride + io;
     double sa = 0.0, sb = 0.0;
 #pragma unroll
     for (int q = 0; q < CQ; q++) {
@@ -2436,7 +2440,10 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
   __shared__ double xp[kPS];
   __shared__ double part[2][NW][kWave];
   const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
+  // everything per node depends on p alone: one round of loads
   const int m = P.m[p], npiv = P.npiv[p];
+  const T* L = front_ptr<T>(P, p);
+  const long long pio = P.pinvoff[p], io = P.ioff[p];
   const int q = (npiv + kPS - 1) / kPS - 1 - s;
   if (q < 0) return;
   const int b0 = q * kPS, w = min(kPS, npiv - b0);
@@ -2444,14 +2451,13 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
   const int c0 = tile * kStepCols;
   if (tile > 0 && c0 >= b0) return;
   const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
-  const T* L = front_ptr<T>(P, p);
-  const double* base = P.pinv + P.pinvoff[p] + (long long)q * kPinvPair;
+  const double* base = P.pinv + pio + (long long)q * kPinvPair;
   const double* invaT = base + kPB * kPB;
   const double* invbT = base + kPinvPanel + kPB * kPB;
   const double* XT = base + 2 * kPinvPanel + kPB * kPB;
   const int cmax = max(b0 - 1, 0);
   // a 2x2 pivot split by the step boundary: L[b0, b0 - 1] is not part of L
-  const bool cut = b0 > 0 && P.ptype[P.ioff[p] + b0 - 1] == 2;
+  const bool cut = b0 > 0 && P.ptype[io + b0 - 1] == 2;
   double lcol[CPW][2], ta[CQ], tx[CQ], tb[CQ];
 #pragma unroll
   for (int k = 0; k < CPW; k++) {
@@ -2470,10 +2476,10 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
     tx[k] = i < wb ? XT[lane + i * kPB] : 0.0;
     tb[k] = i < wb ? invbT[lane + i * kPB] : 0.0;
   }
-  const int* vars = P.vars + P.ioff[p];
+  const int* vars = P.vars + io;
   {
     const int r = blockIdx.y;  // one right-hand side per grid row
-    double* y = Y + (long long)r * ystride + P.ioff[p];
+    double* y = Y + (long long)r * ystride + io;
     double sa = 0.0, sb = 0.0;
 #pragma unroll
     for (int k = 0; k < CQ; k++) {
