@@ -980,7 +980,7 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
 // panel_factor's (mode 0 threshold with delays, mode 1 Bunch-Kaufman) on the
 // same FP64 values; L entries are formed with explicit FMAs.
 // --------------------------------------------------------------------------
-constexpr int kWCRows = kNBLarge + 2;
+constexpr int kWCRows = 2 * kNBLarge;  // W rows cached in LDS (panel positions and swap partners)
 
 // Pivot arithmetic shared by the sequential and the speculative panel paths,
 // evaluated without contraction so that both make the same decisions.
@@ -1008,7 +1008,7 @@ struct RRShared {
   double pr[2][NW];  //   max |v| over the rows below the candidates
   double qr[2][NW];  // row reduction of the imax column
   double wkk[2], fk1k[2], wik[2], wii[2];
-  double wc[kWCRows][kNBLarge];  // W rows of positions kstart .. kstart + kWCRows - 1
+  double wc[kWCRows][kNBLarge + 1];  // W rows of positions kstart .. kstart + kWCRows - 1 (padded)
   double xw[kNBLarge];           // W row of a general imax
   double xa[kNBLarge + 2], xb[kNBLarge + 2];
   double swii, swik;
@@ -1596,12 +1596,15 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
             }
           }
         } else {
-          // general imax: its W row from HBM (written by its owner), its
-          // column (row imax left of the diagonal) from the front
+          // general imax: its W row from the LDS cache or HBM (written by
+          // its owner), its column (row imax left of the diagonal) from the
+          // front
+          const bool wcached = imax - kstart < kWCRows;
 #pragma unroll
           for (int j = 0; j < R; j++)
             if (rowof(j) == imax) {
-              for (int c = 0; c < kw; c++) sh.xw[c] = Wg[imax + (size_t)c * ldw];
+              if (!wcached)
+                for (int c = 0; c < kw; c++) sh.xw[c] = Wg[imax + (size_t)c * ldw];
               sh.swik = v[j];
             }
 #pragma unroll
@@ -1610,7 +1613,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
             vp[j] = (double)(r < imax ? F[imax + (size_t)r * ld] : F[r + (size_t)imax * ld]);
           }
           lds_sync();
-          wr2 = sh.xw;
+          wr2 = wcached ? sh.wc[imax - kstart] : sh.xw;
           for (int c = 0; c < kw; c++) {
             const double w = wr2[c];
 #pragma unroll
@@ -1665,12 +1668,55 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
       step = __builtin_amdgcn_readfirstlane(step);
       kpiv = __builtin_amdgcn_readfirstlane(kpiv);
       const bool fast = step > 0 && kpiv == k + step - 1;
+      // a swap of two positions whose W rows are in the LDS cache is done
+      // there (and written back), without a round trip through HBM
+      bool wswap = false;
       if (!fast) {
         // delay (step 0), 1x1 at imax, or 2x2 whose partner is not k+1
+        const int a = (step == 0) ? k : k + step - 1;
+        const int b = (step == 0) ? nc - 1 : kpiv;
+        const int cnt = kw + step;
+        wswap = a != b && b - kstart < kWCRows;
         if (step == 1) {
 #pragma unroll
           for (int j = 0; j < R; j++) v[j] = vp[j];  // the pivot column is imax
         }
+        if (wswap) {
+          for (int c = tid; c < kw; c += NT) {
+            const double ta = sh.wc[a - kstart][c], tb = sh.wc[b - kstart][c];
+            sh.wc[a - kstart][c] = tb;
+            sh.wc[b - kstart][c] = ta;
+            Wg[a + (size_t)c * ldw] = tb;
+            Wg[b + (size_t)c * ldw] = ta;
+          }
+          for (int c = tid; c < kw; c += NT) {
+            const T tl = lsr(c, a);
+            lsr(c, a) = lsr(c, b);
+            lsr(c, b) = tl;
+          }
+          sym_swap<NT, T>(F, ld, m, vars, pre, a, b);
+#pragma unroll
+          for (int j = 0; j < R; j++) {
+            const int r = rowof(j);
+            if (r == a || r == b) {
+              double* dst = (r == a) ? sh.xa : sh.xb;
+              dst[0] = v[j];
+              dst[1] = vp[j];
+            }
+          }
+          // (a full barrier: the D entries written below share addresses
+          // with the swap's stores)
+          __syncthreads();
+#pragma unroll
+          for (int j = 0; j < R; j++) {
+            const int r = rowof(j);
+            if (r == a || r == b) {
+              const double* src = (r == a) ? sh.xb : sh.xa;
+              v[j] = src[0];
+              vp[j] = src[1];
+            }
+          }
+        } else {
         if (step > 0) {
 #pragma unroll
           for (int j = 0; j < R; j++) {
@@ -1682,9 +1728,6 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
           }
         }
         __syncthreads();
-        const int a = (step == 0) ? k : k + step - 1;
-        const int b = (step == 0) ? nc - 1 : kpiv;
-        const int cnt = kw + step;
         if (a != b) {
           for (int c = tid; c < cnt; c += NT) {
             const double ta = Wg[a + (size_t)c * ldw];
@@ -1723,6 +1766,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
             if (r - kstart < kWCRows) sh.wc[r - kstart][cc] = Wg[r + (size_t)cc * ldw];
           }
         }
+        }
         if (step == 0) nc -= 1;
       }
       if (step == 1) {
@@ -1733,8 +1777,8 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
           const int r = rowof(j);
           if (r >= k && r < m) {
             lsx(kw, j) = zero_col ? (T)v[j] : (T)(v[j] * dinv);
-            if (fast) Wg[r + (size_t)kw * ldw] = v[j];
-            else if (r - kstart < kWCRows) sh.wc[r - kstart][kw] = v[j];
+            if (fast || wswap) Wg[r + (size_t)kw * ldw] = v[j];
+            if (!fast && r - kstart < kWCRows) sh.wc[r - kstart][kw] = v[j];
           }
         }
         if (lane == 0) sh.cT[kw] = zero_col ? 4 : 1;  // every wave writes the same value
@@ -1756,10 +1800,11 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
           if (r >= k && r < m) {
             lsx(kw, j) = (T)__builtin_fma(v[j], i00, vp[j] * i01);
             lsx(kw + 1, j) = (T)__builtin_fma(v[j], i01, vp[j] * i11);
-            if (fast) {
+            if (fast || wswap) {
               Wg[r + (size_t)kw * ldw] = v[j];
               Wg[r + (size_t)(kw + 1) * ldw] = vp[j];
-            } else if (r - kstart < kWCRows) {
+            }
+            if (!fast && r - kstart < kWCRows) {
               sh.wc[r - kstart][kw] = v[j];
               sh.wc[r - kstart][kw + 1] = vp[j];
             }
