@@ -2241,15 +2241,16 @@ constexpr int kPS = 2 * kPB;   // solve step width: two panels
 constexpr long long kPinvPair = 3 * kPinvPanel;
 
 // One wave per (node, panel s): lane c computes column c of B_s^{-1} by
-// forward substitution against B_s held in LDS; the inverse is written
-// column major (lanes = rows) and transposed (lanes = columns), so that both
-// solve directions read it coalesced.  Nodes that failed (stat != 0) are
-// refactored on a new plan and skipped here.
+// forward substitution against B_s held in LDS (row major, read as
+// broadcasts), with the column in registers; the inverse is written
+// transposed (lanes = columns) and, through LDS, column major (lanes =
+// rows), so that both solve directions read it coalesced.  Nodes that failed
+// (stat != 0) are refactored on a new plan and skipped here.
 template <class T>
 __global__ void __launch_bounds__(kWave) k_panel_inv(DevPlan P, const int* inode,
                                                      const int* ipanel) {
-  __shared__ double Bs[kPB * kPB];        // B[i][j] at i + j * kPB
-  __shared__ double Zs[kPB * (kPB + 1)];  // B^{-1}[i][c] at i * (kPB + 1) + c
+  constexpr int LB = kPB + 1;         // padded row of Bs
+  __shared__ double Bs[kPB * LB];     // B[i][j] at i * LB + j; later Z[i][c] at i * LB + c
   const int p = inode[blockIdx.x], s = ipanel[blockIdx.x];
   if (P.stat[p] != 0) return;
   const int m = P.m[p], npiv = P.npiv[p];
@@ -2268,33 +2269,45 @@ __global__ void __launch_bounds__(kWave) k_panel_inv(DevPlan P, const int* inode
       else if (i > j && !(i == j + 1 && ptype[b0 + j] == 2))
         v = (double)L[(size_t)(b0 + i) + (size_t)(b0 + j) * m];
     }
-    Bs[i + j * kPB] = v;
-    Zs[j * (kPB + 1) + lane] = j == lane ? 1.0 : 0.0;
+    Bs[i * LB + j] = v;
   }
   __syncthreads();
-  // z_i = -sum_{j < i} B[i][j] z_j for i > c (z_c = 1, z_j = 0 for j < c);
-  // each lane touches only its own column of Z
-  for (int i = 1; i < w; i++) {
-    double acc = 0.0;
-    for (int j = 0; j < i; j++) acc += Bs[i + j * kPB] * Zs[j * (kPB + 1) + lane];
-    if (i > lane) Zs[i * (kPB + 1) + lane] = -acc;
+  // z_i = -sum_{j < i} B[i][j] z_j for i > c (z_c = 1, z_j = 0 for j < c)
+  double z[kPB];
+#pragma unroll
+  for (int i = 0; i < kPB; i++) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < i; j++) {
+      const double bij = Bs[i * LB + j];
+      if (j & 1) a1 = __builtin_fma(bij, z[j], a1);
+      else a0 = __builtin_fma(bij, z[j], a0);
+    }
+    z[i] = i == lane ? 1.0 : (i > lane ? -(a0 + a1) : 0.0);
   }
-  __syncthreads();
   double* inv = P.pinv + P.pinvoff[p] + (long long)(s / 2) * kPinvPair + (s % 2) * kPinvPanel;
-  for (int c = 0; c < kPB; c++) inv[lane + c * kPB] = Zs[lane * (kPB + 1) + c];
-  for (int i = 0; i < kPB; i++) inv[kPB * kPB + lane + i * kPB] = Zs[i * (kPB + 1) + lane];
+  // transposed copy: row c of B^{-T} is column c of B^{-1}, held by lane c
+#pragma unroll
+  for (int i = 0; i < kPB; i++) inv[kPB * kPB + lane + i * kPB] = z[i];
+  __syncthreads();  // every lane is done with B
+#pragma unroll
+  for (int i = 0; i < kPB; i++) Bs[i * LB + lane] = z[i];
+  __syncthreads();
+  for (int c = 0; c < kPB; c++) inv[lane + c * kPB] = Bs[lane * LB + c];
 }
 
 // Composition of two consecutive 64-column blocks into one 128-column solve
 // step: for B = [B1 0; C B2] (C = L rows of block 2, columns of block 1, with
 // the slot of a 2x2 pivot split by the block boundary taken as zero),
-// B^{-1} = [B1^{-1} 0; X B2^{-1}] with X = -B2^{-1} C B1^{-1}.  One
-// workgroup per (node, block pair) forms X and X^T after k_panel_inv.
+// B^{-1} = [B1^{-1} 0; X B2^{-1}] with X = -B2^{-1} C B1^{-1}.  One wave per
+// (node, block pair) after k_panel_inv: lane c forms column c of C B1^{-1}
+// from column c of B1^{-1} (registers) and C (LDS broadcasts), then column
+// c of X against B2^{-1} (LDS broadcasts), and writes X and X^T.
 template <class T>
-__global__ void __launch_bounds__(kTPB) k_panel_inv2(DevPlan P, const int* inode,
-                                                     const int* ipair) {
-  __shared__ double Cs[kPB * kPB];  // C[i][j] at i + j * kPB
-  __shared__ double Ts[kPB * kPB];  // C B1^{-1}
+__global__ void __launch_bounds__(kWave) k_panel_inv2(DevPlan P, const int* inode,
+                                                      const int* ipair) {
+  constexpr int LB = kPB + 1;
+  __shared__ double S[kPB * LB];  // C[r][j] at r * LB + j; then B2^{-1}; then X
   const int p = inode[blockIdx.x], q = ipair[blockIdx.x];
   if (P.stat[p] != 0) return;
   const int m = P.m[p], npiv = P.npiv[p];
@@ -2305,33 +2318,58 @@ __global__ void __launch_bounds__(kTPB) k_panel_inv2(DevPlan P, const int* inode
   const T* L = front_ptr<T>(P, p);
   const int* ptype = P.ptype + P.ioff[p];
   double* base = P.pinv + P.pinvoff[p] + (long long)q * kPinvPair;
-  const double* inva = base;
-  const double* invb = base + kPinvPanel;
+  const double* invaT = base + kPB * kPB;          // row c = column c of B1^{-1}
+  const double* invb = base + kPinvPanel;          // column major
   double* X = base + 2 * kPinvPanel;
   const bool cut = ptype[bb - 1] == 2;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < kPB * kPB; e += kTPB) {
-    const int i = e % kPB, j = e / kPB;
+  const int lane = threadIdx.x;
+  for (int j = 0; j < kPB; j++) {
     double v = 0.0;
-    if (i < wb && !(cut && i == 0 && j == kPB - 1))
-      v = (double)L[(size_t)(bb + i) + (size_t)(ba + j) * m];
-    Cs[e] = v;
+    if (lane < wb && !(cut && lane == 0 && j == kPB - 1))
+      v = (double)L[(size_t)(bb + lane) + (size_t)(ba + j) * m];
+    S[lane * LB + j] = v;
   }
+  // column c of B1^{-1} (zero above c)
+  double t[kPB];
+#pragma unroll
+  for (int j = 0; j < kPB; j++) t[j] = invaT[lane + j * kPB];
   __syncthreads();
-  for (int e = tid; e < kPB * kPB; e += kTPB) {
-    const int i = e % kPB, c = e / kPB;
-    double a = 0.0;
-    for (int j = c; j < kPB; j++) a += Cs[i + j * kPB] * inva[j + c * kPB];
-    Ts[e] = a;
+  // u = C z (u overwrites z from the top: u[r] needs all of z)
+  double u[kPB];
+#pragma unroll
+  for (int r = 0; r < kPB; r++) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < kPB; j++) {
+      const double crj = S[r * LB + j];
+      if (j & 1) a1 = __builtin_fma(crj, t[j], a1);
+      else a0 = __builtin_fma(crj, t[j], a0);
+    }
+    u[r] = a0 + a1;
   }
+  __syncthreads();  // C no longer needed
+  for (int i = 0; i < kPB; i++) S[i * LB + lane] = invb[i + lane * kPB];  // B2^{-1}[i][lane]
   __syncthreads();
-  for (int e = tid; e < kPB * kPB; e += kTPB) {
-    const int i = e % kPB, c = e / kPB;
-    double a = 0.0;
-    for (int r = 0; r <= i; r++) a += invb[i + r * kPB] * Ts[r + c * kPB];
-    X[i + c * kPB] = -a;
-    X[kPB * kPB + c + i * kPB] = -a;
+  // column c of X = -B2^{-1} u (B2^{-1} lower triangular)
+#pragma unroll
+  for (int i = 0; i < kPB; i++) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int r = 0; r <= i; r++) {
+      const double bir = S[i * LB + r];
+      if (r & 1) a1 = __builtin_fma(bir, u[r], a1);
+      else a0 = __builtin_fma(bir, u[r], a0);
+    }
+    t[i] = -(a0 + a1);
   }
+  // X^T (row c = column c of X, held by lane c), then X column major through LDS
+#pragma unroll
+  for (int i = 0; i < kPB; i++) X[kPB * kPB + lane + i * kPB] = t[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kPB; i++) S[i * LB + lane] = t[i];
+  __syncthreads();
+  for (int c = 0; c < kPB; c++) X[lane + c * kPB] = S[lane * LB + c];
 }
 
 // Forward step s of the large nodes of a level: the 128 columns
@@ -3349,7 +3387,7 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
     hipLaunchKernelGGL(k_panel_inv<T>, dim3((unsigned)H.inv_node.size()), dim3(kWave), 0, st, P,
                        H.d_inv_node.p, H.d_inv_panel.p);
     if (!H.inv2_node.empty())
-      hipLaunchKernelGGL(k_panel_inv2<T>, dim3((unsigned)H.inv2_node.size()), dim3(kTPB), 0, st,
+      hipLaunchKernelGGL(k_panel_inv2<T>, dim3((unsigned)H.inv2_node.size()), dim3(kWave), 0, st,
                          P, H.d_inv2_node.p, H.d_inv2_pair.p);
   }
   HIPCHECK(hipGetLastError());
