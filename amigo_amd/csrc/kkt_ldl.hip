@@ -2235,6 +2235,7 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
 constexpr int kPB = 64;        // panel width of the partitioned inverse
 constexpr int kStepRows = 64;  // forward step: rows per workgroup (one per lane)
 constexpr int kStepCols = 32;  // backward step: columns per workgroup (8 per wave)
+constexpr int kStepRhs = 8;    // right-hand sides per workgroup of a step (operands reused)
 constexpr long long kPinvPanel = 2LL * kPB * kPB;  // doubles per panel: B^{-1}, B^{-T}
 constexpr int kPS = 2 * kPB;   // solve step width: two panels
 // doubles per solve step: both panels' B^{-1}, B^{-T}, then X and X^T (below)
@@ -2434,8 +2435,9 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   const int* vars = P.vars + io;
   const double* dv = P.dv + io;
   const double* dv2 = P.dv2 + io;
-  {
-    const int r = blockIdx.y;  // one right-hand side per grid row
+  // kStepRhs right-hand sides per grid row, on the operands loaded once
+  const int r1 = min(nrhs, ((int)blockIdx.y + 1) * kStepRhs);
+  for (int r = blockIdx.y * kStepRhs; r < r1; r++) {
     double* y = Y + (long long)r * ystride + io;
     double sa = 0.0, sb = 0.0;
 #pragma unroll
@@ -2560,8 +2562,9 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
     tb[k] = i < wb ? invbT[lane + i * kPB] : 0.0;
   }
   const int* vars = P.vars + io;
-  {
-    const int r = blockIdx.y;  // one right-hand side per grid row
+  // kStepRhs right-hand sides per grid row, on the operands loaded once
+  const int r1 = min(nrhs, ((int)blockIdx.y + 1) * kStepRhs);
+  for (int r = blockIdx.y * kStepRhs; r < r1; r++) {
     double* y = Y + (long long)r * ystride + io;
     double sa = 0.0, sb = 0.0;
 #pragma unroll
@@ -3549,7 +3552,8 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
       for (int t = 0; t < nst; t++) {
         const int slot = H.fs_lvl[l] + t;
         const int f0 = H.fs_ptr[slot], f1 = H.fs_ptr[slot + 1];
-        hipLaunchKernelGGL(k_fwd_step<T>, dim3(f1 - f0, nrhs), dim3(kTPB), 0, st, P,
+        hipLaunchKernelGGL(k_fwd_step<T>, dim3(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs),
+                           dim3(kTPB), 0, st, P,
                            H.d_fs_node.p + f0, H.d_fs_tile.p + f0, t, dx, ldx, nrhs, us, Y, ys);
       }
     }
@@ -3565,7 +3569,8 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
       for (int t = 0; t < nst; t++) {
         const int slot = H.bs_lvl[l] + t;
         const int f0 = H.bs_ptr[slot], f1 = H.bs_ptr[slot + 1];
-        hipLaunchKernelGGL(k_bwd_step<T>, dim3(f1 - f0, nrhs), dim3(kTPB), 0, st, P,
+        hipLaunchKernelGGL(k_bwd_step<T>, dim3(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs),
+                           dim3(kTPB), 0, st, P,
                            H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
       }
     }
