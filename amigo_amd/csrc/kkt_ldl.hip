@@ -2812,6 +2812,10 @@ struct Handle {
   std::vector<int> h_npiv, h_ndel, h_cnt, h_stat;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   float last_factor_ms = 0.f, last_solve_ms = 0.f;
+  // second stream (default priority) for the small fronts of a level, which
+  // run beside its large-front chain; fork and join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 static void build_plan(Handle& H, const std::vector<int>& slack) {
@@ -3330,18 +3334,29 @@ template <class T>
 static void run_factor_levels_t(Handle& H, hipStream_t st) {
   DevPlan P = dev_plan(H);
   const int nlev = (int)H.lvl_ptr.size() - 1;
+  bool joined = true;  // the side stream's work is ordered before st's next launch
   for (int l = 0; l < nlev; l++) {
     int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
+    if (!joined) {
+      HIPCHECK(hipStreamWaitEvent(st, H.ev_join, 0));
+      joined = true;
+    }
     {
       ProfScope ps_(H, st, KC_PREPARE);
       hipLaunchKernelGGL(k_prepare, dim3((n1 - n0 + kTPB - 1) / kTPB), dim3(kTPB), 0, st,
                          P, H.d_lvl_nodes.p + n0, n1 - n0);
     }
-    int a0 = H.asm_lvl_ptr[l], a1 = H.asm_lvl_ptr[l + 1];
-    if (a1 > a0) {
-      ProfScope ps_(H, st, KC_ASSEMBLE);
-      hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
-                         H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
+    // The level's LDS fronts (assembled in LDS) depend only on k_prepare
+    // and on the levels below; when the level also has fronts assembled in
+    // HBM they run on the side stream, beside that assembly and those
+    // fronts' factorization, and the next level waits for both.
+    const bool fork = H.lds_ptr[l + 1] > H.lds_ptr[l] &&
+                      (H.large_ptr[l + 1] > H.large_ptr[l] || H.small_ptr[l + 1] > H.small_ptr[l]);
+    hipStream_t ss = st;
+    if (fork) {
+      HIPCHECK(hipEventRecord(H.ev_fork, st));
+      HIPCHECK(hipStreamWaitEvent(H.side, H.ev_fork, 0));
+      ss = H.side;
     }
     // LDS fronts in size buckets, so that small fronts get small LDS
     // allocations and several workgroups per CU
@@ -3352,10 +3367,20 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       int d1 = d0;
       while (d1 < H.lds_ptr[l + 1] && H.cap[H.lds_nodes[d1]] <= bucket) d1++;
       const size_t bytes = lds_front_bytes(bucket, sizeof(T));
-      ProfScope ps_(H, st, KC_FRONT_LDS);
-      hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(d1 - d0), dim3(kTPB), bytes, st, P,
+      ProfScope ps_(H, ss, KC_FRONT_LDS);
+      hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(d1 - d0), dim3(kTPB), bytes, ss, P,
                          H.d_lds_nodes.p + d0);
       d0 = d1;
+    }
+    if (fork) {
+      HIPCHECK(hipEventRecord(H.ev_join, H.side));
+      joined = false;
+    }
+    int a0 = H.asm_lvl_ptr[l], a1 = H.asm_lvl_ptr[l + 1];
+    if (a1 > a0) {
+      ProfScope ps_(H, st, KC_ASSEMBLE);
+      hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
+                         H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
     }
     const int s0 = H.small_ptr[l], s1 = H.small_ptr[l + 1];
     if (s1 > s0) {
@@ -3385,6 +3410,7 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       }
     }
   }
+  if (!joined) HIPCHECK(hipStreamWaitEvent(st, H.ev_join, 0));
   if (!H.inv_node.empty()) {
     ProfScope ps_(H, st, KC_INVERT);
     hipLaunchKernelGGL(k_panel_inv<T>, dim3((unsigned)H.inv_node.size()), dim3(kWave), 0, st, P,
@@ -3422,6 +3448,9 @@ static void ensure_device(Handle& H) {
   HIPCHECK(hipEventCreate(&H.ev1));
   HIPCHECK(hipEventCreate(&H.ev2));
   HIPCHECK(hipEventCreate(&H.ev3));
+  HIPCHECK(hipStreamCreateWithFlags(&H.side, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreateWithFlags(&H.ev_fork, hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&H.ev_join, hipEventDisableTiming));
   H.uploaded = true;
 }
 
@@ -3923,6 +3952,9 @@ void kkt_ldl_destroy(kkt_ldl_t h) {
   if (H.ev1) (void)hipEventDestroy(H.ev1);
   if (H.ev2) (void)hipEventDestroy(H.ev2);
   if (H.ev3) (void)hipEventDestroy(H.ev3);
+  if (H.ev_fork) (void)hipEventDestroy(H.ev_fork);
+  if (H.ev_join) (void)hipEventDestroy(H.ev_join);
+  if (H.side) (void)hipStreamDestroy(H.side);
   delete h;
 }
 
