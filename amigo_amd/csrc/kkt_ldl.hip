@@ -1942,7 +1942,10 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
   // output row of register r in this lane
   auto orow = [&](int r) { return f64 ? lk + 4 * r : 4 * lk + r; };
   // Accumulator starts from the F tile; the A operand is -L, so the MFMA
-  // chain produces F - L W^T directly.  All loads are issued up front.
+  // chain produces F - L W^T directly.  All loads are issued up front, from
+  // addresses clamped into the front (and into W's kNBLarge columns), and
+  // masked afterwards: a load under a condition becomes a branch, and the
+  // compiler then waits for each such load before issuing the next.
   vec4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -1950,22 +1953,33 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = ri + 16 * a + orow(r);
-        const int j = cj + 16 * b + lr;
-        acc[a][b][r] = (i < m && j < m) ? F[i + (size_t)j * m] : (T)0;
+        const int i = min(ri + 16 * a + orow(r), m - 1);
+        const int j = min(cj + 16 * b + lr, m - 1);
+        acc[a][b][r] = F[i + (size_t)j * m];
       }
   constexpr int KS = kNBLarge / 4;  // k-steps of 4 (kw <= kNBLarge)
   T av[KS][2], bv[KS][2];
+  double wv[KS][2];
 #pragma unroll
   for (int q = 0; q < KS; q++) {
     const int c = 4 * q + lk;
-    const bool ok = c < kw;
 #pragma unroll
     for (int a = 0; a < 2; a++) {
-      const int row = ri + 16 * a + lr;
-      av[q][a] = (ok && row < m) ? -F[row + (size_t)(kstart + c) * m] : (T)0;
-      const int col = cj + 16 * a + lr;
-      bv[q][a] = (ok && col < m) ? (T)W[col + (size_t)c * ldw] : (T)0;
+      const int row = min(ri + 16 * a + lr, m - 1);
+      av[q][a] = F[row + (size_t)min(kstart + c, m - 1) * m];
+      const int col = min(cj + 16 * a + lr, m - 1);
+      wv[q][a] = W[col + (size_t)c * ldw];
+    }
+  }
+  // rows and columns of the tile beyond m only reach outputs that are not
+  // stored; the k columns beyond kw must be zero
+#pragma unroll
+  for (int q = 0; q < KS; q++) {
+    const bool ok = 4 * q + lk < kw;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      av[q][a] = ok ? -av[q][a] : (T)0;
+      bv[q][a] = ok ? (T)wv[q][a] : (T)0;
     }
   }
 #pragma unroll
