@@ -1607,12 +1607,17 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
                 for (int c = 0; c < kw; c++) sh.xw[c] = Wg[imax + (size_t)c * ldw];
               sh.swik = v[j];
             }
+          // all R loads in flight before the barrier; converted after it
+          T raw[R];
 #pragma unroll
           for (int j = 0; j < R; j++) {
             const int r = min(rowof(j), m - 1);
-            vp[j] = (double)(r < imax ? F[imax + (size_t)r * ld] : F[r + (size_t)imax * ld]);
+            raw[j] = r < imax ? F[imax + (size_t)r * ld] : F[r + (size_t)imax * ld];
           }
+          __builtin_amdgcn_sched_barrier(0);
           lds_sync();
+#pragma unroll
+          for (int j = 0; j < R; j++) vp[j] = (double)raw[j];
           wr2 = wcached ? sh.wc[imax - kstart] : sh.xw;
           for (int c = 0; c < kw; c++) {
             const double w = wr2[c];
@@ -2276,15 +2281,27 @@ __global__ void __launch_bounds__(kWave) k_panel_inv(DevPlan P, const int* inode
   const T* L = front_ptr<T>(P, p);
   const int* ptype = P.ptype + P.ioff[p];
   const int lane = threadIdx.x;
-  for (int j = 0; j < kPB; j++) {
-    const int i = lane;
-    double v = 0.0;
-    if (i < w && j < w) {
-      if (i == j) v = 1.0;
-      else if (i > j && !(i == j + 1 && ptype[b0 + j] == 2))
-        v = (double)L[(size_t)(b0 + i) + (size_t)(b0 + j) * m];
+  // 16 columns at a time, every load issued before the first use (from
+  // addresses clamped into the front, masked afterwards)
+  const size_t ri = (size_t)min(b0 + lane, m - 1);
+  for (int j0 = 0; j0 < kPB; j0 += 16) {
+    T raw[16];
+    int pt[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      raw[jj] = L[ri + (size_t)min(b0 + j0 + jj, m - 1) * m];
+      pt[jj] = ptype[min(b0 + j0 + jj, npiv - 1)];
     }
-    Bs[i * LB + j] = v;
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      const int i = lane, j = j0 + jj;
+      double v = 0.0;
+      if (i < w && j < w) {
+        if (i == j) v = 1.0;
+        else if (i > j && !(i == j + 1 && pt[jj] == 2)) v = (double)raw[jj];
+      }
+      Bs[i * LB + j] = v;
+    }
   }
   __syncthreads();
   // z_i = -sum_{j < i} B[i][j] z_j for i > c (z_c = 1, z_j = 0 for j < c)
@@ -2338,11 +2355,18 @@ __global__ void __launch_bounds__(kWave) k_panel_inv2(DevPlan P, const int* inod
   double* X = base + 2 * kPinvPanel;
   const bool cut = ptype[bb - 1] == 2;
   const int lane = threadIdx.x;
-  for (int j = 0; j < kPB; j++) {
-    double v = 0.0;
-    if (lane < wb && !(cut && lane == 0 && j == kPB - 1))
-      v = (double)L[(size_t)(bb + lane) + (size_t)(ba + j) * m];
-    S[lane * LB + j] = v;
+  // 16 columns at a time, every load issued before the first use
+  const size_t rb = (size_t)min(bb + lane, m - 1);
+  for (int j0 = 0; j0 < kPB; j0 += 16) {
+    T raw[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) raw[jj] = L[rb + (size_t)(ba + j0 + jj) * m];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      const int j = j0 + jj;
+      S[lane * LB + j] =
+          (lane < wb && !(cut && lane == 0 && j == kPB - 1)) ? (double)raw[jj] : 0.0;
+    }
   }
   // column c of B1^{-1} (zero above c)
   double t[kPB];
@@ -2363,7 +2387,13 @@ __global__ void __launch_bounds__(kWave) k_panel_inv2(DevPlan P, const int* inod
     u[r] = a0 + a1;
   }
   __syncthreads();  // C no longer needed
-  for (int i = 0; i < kPB; i++) S[i * LB + lane] = invb[i + lane * kPB];  // B2^{-1}[i][lane]
+  for (int i0 = 0; i0 < kPB; i0 += 16) {  // B2^{-1}[i][lane], 16 loads in flight
+    double raw[16];
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) raw[ii] = invb[i0 + ii + lane * kPB];
+#pragma unroll
+    for (int ii = 0; ii < 16; ii++) S[(i0 + ii) * LB + lane] = raw[ii];
+  }
   __syncthreads();
   // column c of X = -B2^{-1} u (B2^{-1} lower triangular)
 #pragma unroll
