@@ -1951,17 +1951,26 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
   // addresses clamped into the front (and into W's kNBLarge columns), and
   // masked afterwards: a load under a condition becomes a branch, and the
   // compiler then waits for each such load before issuing the next.
+  // FP32: a lane's four accumulator rows are consecutive in the column, one
+  // 16-byte access (4-byte aligned; reading at most 3 entries past the
+  // front's last one, inside the arena's padding)
+  typedef T vec4u __attribute__((ext_vector_type(4), aligned(sizeof(T))));
   vec4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < 2; b++) {
+      const int j = min(cj + 16 * b + lr, m - 1);
+      if constexpr (!f64) {
+        const int i = min(ri + 16 * a + 4 * lk, m - 1);
+        const vec4u v = *reinterpret_cast<const vec4u*>(F + i + (size_t)j * m);
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = min(ri + 16 * a + orow(r), m - 1);
-        const int j = min(cj + 16 * b + lr, m - 1);
-        acc[a][b][r] = F[i + (size_t)j * m];
+        for (int r = 0; r < 4; r++) acc[a][b][r] = v[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[a][b][r] = F[min(ri + 16 * a + orow(r), m - 1) + (size_t)j * m];
       }
+    }
   constexpr int KS = kNBLarge / 4;  // k-steps of 4 (kw <= kNBLarge)
   T av[KS][2], bv[KS][2];
   double wv[KS][2];
@@ -2003,13 +2012,30 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < 2; b++) {
+      const int j = cj + 16 * b + lr;
+      if constexpr (!f64) {
+        const int i = ri + 16 * a + 4 * lk;
+        if (j < m && j >= k) {
+          if (i >= j && i + 3 < m) {
+            vec4u v;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int i = ri + 16 * a + orow(r);
-        const int j = cj + 16 * b + lr;
-        if (i < m && j < m && i >= j && j >= k) F[i + (size_t)j * m] = acc[a][b][r];
+            for (int r = 0; r < 4; r++) v[r] = acc[a][b][r];
+            *reinterpret_cast<vec4u*>(F + i + (size_t)j * m) = v;
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+              if (i + r < m && i + r >= j) F[i + r + (size_t)j * m] = acc[a][b][r];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = ri + 16 * a + orow(r);
+          if (i < m && j < m && i >= j && j >= k) F[i + (size_t)j * m] = acc[a][b][r];
+        }
       }
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -3207,7 +3233,9 @@ static void upload_static(Handle& H) {
   H.d_inv2_pair.upload(H.inv2_pair);
   H.d_pinv.alloc((size_t)H.pinv_total);
   H.y_nrhs = 0;
-  H.d_F.alloc(((size_t)H.f_total * H.esize + 7) / 8);
+  // 64 bytes of padding: the trailing update's 16-byte tile reads can pass
+  // the last front's last entry by up to 12 bytes
+  H.d_F.alloc(((size_t)H.f_total * H.esize + 7) / 8 + 8);
   H.d_dv.alloc((size_t)H.i_total);
   H.d_dv2.alloc((size_t)H.i_total);
   H.d_vars.alloc((size_t)H.i_total);
