@@ -1916,7 +1916,8 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
 }
 
 // FP32 fronts use v_mfma_f32_16x16x4f32 (FP32 accumulate), whose output is
-// D[4(l/16) + r][l%16] (also checked on gfx950).
+// D[4(l/16) + r][l%16] (also checked on gfx950); FP64 fronts
+// v_mfma_f64_16x16x4f64, whose output is D[l/16 + 4r][l%16].
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
                                                  const int* wtile, int base) {
@@ -1944,16 +1945,21 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
   if (ri + 32 <= cj) return;  // strictly above the diagonal
   const int lr = l & 15, lk = l >> 4;
   constexpr bool f64 = sizeof(T) == 8;
-  // output row of register r in this lane
-  auto orow = [&](int r) { return f64 ? lk + 4 * r : 4 * lk + r; };
+  // Register r of lane (lk, lr) holds the tile entry at row 4 lk + r,
+  // column lr of its 16x16 subtile, so a lane's four rows are consecutive
+  // in the column: one 16-byte (FP32) or two (FP64) accesses per subtile.
+  // FP32 MFMA output row 4 (l/16) + r is that row already.  The FP64 MFMA
+  // outputs row l/16 + 4 r; its A operand therefore supplies, in MFMA row
+  // x = q + 4 s, the L row 4 q + s, which moves output row l/16 + 4 r to
+  // row 4 (l/16) + r.
+  const int arow = f64 ? 4 * (lr & 3) + (lr >> 2) : lr;
   // Accumulator starts from the F tile; the A operand is -L, so the MFMA
   // chain produces F - L W^T directly.  All loads are issued up front, from
   // addresses clamped into the front (and into W's kNBLarge columns), and
   // masked afterwards: a load under a condition becomes a branch, and the
-  // compiler then waits for each such load before issuing the next.
-  // FP32: a lane's four accumulator rows are consecutive in the column, one
-  // 16-byte access (4-byte aligned; reading at most 3 entries past the
-  // front's last one, inside the arena's padding)
+  // compiler then waits for each such load before issuing the next.  The
+  // tile reads can pass the front's last entry by up to 3 entries, inside
+  // the arena's padding.
   typedef T vec4u __attribute__((ext_vector_type(4), aligned(sizeof(T))));
   vec4 acc[2][2];
 #pragma unroll
@@ -1961,15 +1967,10 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
 #pragma unroll
     for (int b = 0; b < 2; b++) {
       const int j = min(cj + 16 * b + lr, m - 1);
-      if constexpr (!f64) {
-        const int i = min(ri + 16 * a + 4 * lk, m - 1);
-        const vec4u v = *reinterpret_cast<const vec4u*>(F + i + (size_t)j * m);
+      const int i = min(ri + 16 * a + 4 * lk, m - 1);
+      const vec4u v = *reinterpret_cast<const vec4u*>(F + i + (size_t)j * m);
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc[a][b][r] = v[r];
-      } else {
-#pragma unroll
-        for (int r = 0; r < 4; r++) acc[a][b][r] = F[min(ri + 16 * a + orow(r), m - 1) + (size_t)j * m];
-      }
+      for (int r = 0; r < 4; r++) acc[a][b][r] = v[r];
     }
   constexpr int KS = kNBLarge / 4;  // k-steps of 4 (kw <= kNBLarge)
   T av[KS][2], bv[KS][2];
@@ -1979,7 +1980,7 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
     const int c = 4 * q + lk;
 #pragma unroll
     for (int a = 0; a < 2; a++) {
-      const int row = min(ri + 16 * a + lr, m - 1);
+      const int row = min(ri + 16 * a + arow, m - 1);
       av[q][a] = F[row + (size_t)min(kstart + c, m - 1) * m];
       const int col = min(cj + 16 * a + lr, m - 1);
       wv[q][a] = W[col + (size_t)c * ldw];
@@ -2014,25 +2015,17 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
 #pragma unroll
     for (int b = 0; b < 2; b++) {
       const int j = cj + 16 * b + lr;
-      if constexpr (!f64) {
-        const int i = ri + 16 * a + 4 * lk;
-        if (j < m && j >= k) {
-          if (i >= j && i + 3 < m) {
-            vec4u v;
+      const int i = ri + 16 * a + 4 * lk;
+      if (j < m && j >= k) {
+        if (i >= j && i + 3 < m) {
+          vec4u v;
 #pragma unroll
-            for (int r = 0; r < 4; r++) v[r] = acc[a][b][r];
-            *reinterpret_cast<vec4u*>(F + i + (size_t)j * m) = v;
-          } else {
+          for (int r = 0; r < 4; r++) v[r] = acc[a][b][r];
+          *reinterpret_cast<vec4u*>(F + i + (size_t)j * m) = v;
+        } else {
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-              if (i + r < m && i + r >= j) F[i + r + (size_t)j * m] = acc[a][b][r];
-          }
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int i = ri + 16 * a + orow(r);
-          if (i < m && j < m && i >= j && j >= k) F[i + (size_t)j * m] = acc[a][b][r];
+          for (int r = 0; r < 4; r++)
+            if (i + r < m && i + r >= j) F[i + r + (size_t)j * m] = acc[a][b][r];
         }
       }
     }
