@@ -224,6 +224,19 @@ __device__ inline int child_row(const DevPlan& P, const CV& cv, int i, int ns, i
   return q >= 0 ? cv.fs + q : -1;
 }
 
+// child_row without branches: the map entry is loaded for every position
+// (index clamped into the map), so that several lookups can be in flight
+template <class CV>
+__device__ inline int child_row_nb(const DevPlan& P, const CV& cv, int i, int ns, int din) {
+  const long long inv = cv.inv;
+  const int fs = cv.fs, npiv = cv.npiv, ndel = cv.ndel, doff = cv.doff;
+  const int q = P.invnom[inv + (i < ns ? i : max(i - din, 0))];
+  const int d = i - ns - doff;
+  const int rd = (unsigned)d < (unsigned)ndel ? npiv + d : -1;
+  const int rq = q >= 0 ? fs + q : -1;
+  return (i >= ns && i < ns + din) ? rd : rq;
+}
+
 // --------------------------------------------------------------------------
 // Per-level preparation: delayed pivots received from the children, their
 // offsets inside this front, the actual front dimension, overflow check.
@@ -320,13 +333,27 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
       const int nl = s_nl;
       for (int i = j + tid; i < m; i += kTPB) {
         double v = 0.0;
-        for (int t = 0; t < nl; t++) {
-          const ChildView<T>& cv = s_cv[s_list[t]];
-          const int ri = child_row(P, cv, i, ns, din);
-          if (ri >= 0) {
-            const int rj = s_rj[t];
-            v += (double)cv.F[max(ri, rj) + (size_t)min(ri, rj) * cv.m];
+        // four children at a time: the map lookups, then the entries, all
+        // in flight together (a child without row i reads its entry at row
+        // 0 and does not add it); the sum stays in child order
+        for (int t0 = 0; t0 < nl; t0 += 4) {
+          int ri[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int r = child_row_nb(P, s_cv[s_list[min(t0 + u, nl - 1)]], i, ns, din);
+            ri[u] = t0 + u < nl ? r : -1;
           }
+          T x[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int t = min(t0 + u, nl - 1);
+            const ChildView<T>& cv = s_cv[s_list[t]];
+            const int a = max(ri[u], 0), rj = s_rj[t];
+            x[u] = cv.F[max(a, rj) + (size_t)min(a, rj) * cv.m];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (ri[u] >= 0) v += (double)x[u];
         }
         Fj[i] = (T)v;
       }
