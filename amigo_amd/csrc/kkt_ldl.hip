@@ -69,7 +69,7 @@ struct Error : std::runtime_error {
 constexpr int kNB = 64;        // panel width bound (reference block_size)
 constexpr int kTPB = 256;      // threads per workgroup
 constexpr int kWave = 64;
-constexpr int kAsmCols = 8;    // columns per assembly workgroup
+constexpr int kAsmCols = 4;    // columns per assembly workgroup
 constexpr int kPanelThreads = 1024;  // workgroup of the large-front panel
 constexpr int kNBLarge = 16;    // panel width of large fronts (narrow: the panel re-reads it per column)
 constexpr int kLdsFront = 96;   // default bound of fronts assembled and factored in LDS
