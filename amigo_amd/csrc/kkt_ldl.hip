@@ -2717,28 +2717,59 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   {
     const int r = blockIdx.y;  // one right-hand side per grid row
     const double* xr = x + (size_t)r * ldx;
+    // both index loads issued (clamped into their arrays; a node without
+    // contribution rows reads vars twice), then the gather
+    const int cs = P.cs[p];
+    const int* crow = cs > 0 ? P.contrib_rows + cp : vars;
     for (int q = tid; q < nr; q += kTPB) {
       const int i = npiv + q;
-      xs[q] = xr[i < fs ? vars[i] : P.contrib_rows[cp + i - fs]];
+      const int iv = vars[min(i, fs - 1)];
+      const int ic = crow[min(max(i - fs, 0), max(cs - 1, 0))];
+      xs[q] = xr[i < fs ? iv : ic];
     }
     __syncthreads();
-    for (int jj = 0; jj < kGemvCols / 4; jj++) {
-      const int j = j0 + 4 * w + jj;
-      if (j >= npiv) break;
-      const T* Lj = L + (size_t)j * m + npiv;
-      double s = 0.0;
-      int q = lane;
-      for (; q + 3 * kWave < nr; q += 4 * kWave) {
-        const double a0 = (double)Lj[q], a1 = (double)Lj[q + kWave];
-        const double a2 = (double)Lj[q + 2 * kWave], a3 = (double)Lj[q + 3 * kWave];
-        s = __builtin_fma(a0, xs[q], s);
-        s = __builtin_fma(a1, xs[q + kWave], s);
-        s = __builtin_fma(a2, xs[q + 2 * kWave], s);
-        s = __builtin_fma(a3, xs[q + 3 * kWave], s);
+    // the wave's four columns together, so that their loads are in flight
+    // at once (columns past npiv read column npiv - 1 and are not written);
+    // each column accumulates over q = lane, lane + 64, ... in order
+    constexpr int CW = kGemvCols / 4;
+    const int jb = j0 + 4 * w;
+    const T* Lc[CW];
+#pragma unroll
+    for (int jj = 0; jj < CW; jj++) Lc[jj] = L + (size_t)min(jb + jj, npiv - 1) * m + npiv;
+    double sc[CW];
+#pragma unroll
+    for (int jj = 0; jj < CW; jj++) sc[jj] = 0.0;
+    int q = lane;
+    for (; q + kWave < nr; q += 2 * kWave) {
+      T a[CW][2];
+#pragma unroll
+      for (int jj = 0; jj < CW; jj++) {
+        a[jj][0] = Lc[jj][q];
+        a[jj][1] = Lc[jj][q + kWave];
       }
-      for (; q < nr; q += kWave) s = __builtin_fma((double)Lj[q], xs[q], s);
-      for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) Y[(long long)r * ystride + P.ioff[p] + j] = xr[unpiv(vars[j])] - s;
+      const double x0 = xs[q], x1 = xs[q + kWave];
+#pragma unroll
+      for (int jj = 0; jj < CW; jj++) {
+        sc[jj] = __builtin_fma((double)a[jj][0], x0, sc[jj]);
+        sc[jj] = __builtin_fma((double)a[jj][1], x1, sc[jj]);
+      }
+    }
+    if (q < nr) {
+      const double x0 = xs[q];
+#pragma unroll
+      for (int jj = 0; jj < CW; jj++) sc[jj] = __builtin_fma((double)Lc[jj][q], x0, sc[jj]);
+    }
+#pragma unroll
+    for (int jj = 0; jj < CW; jj++)
+      for (int off = kWave / 2; off > 0; off >>= 1) sc[jj] += __shfl_xor(sc[jj], off);
+    // every lane holds the sums; lane jj writes column jj
+    if (lane < CW) {
+      double v = sc[0];
+#pragma unroll
+      for (int jj = 1; jj < CW; jj++)
+        if (lane == jj) v = sc[jj];
+      const int j = jb + lane;
+      if (j < npiv) Y[(long long)r * ystride + P.ioff[p] + j] = xr[unpiv(vars[j])] - v;
     }
     __syncthreads();
   }
