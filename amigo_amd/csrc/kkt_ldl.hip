@@ -2112,19 +2112,36 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
     double* xr = x + (size_t)r * ldx;
     for (int i = tid; i < m; i += NTH) {
       const int pi = i < fs ? pre[i] : i;
-      double v = pi < ns ? xr[P.node_to_var[vp + pi]] : 0.0;
-      for (int q = 0; q < nch; q++) {
-        ChildView<T> cv;
-        long long uo;
-        if (cached) {
-          cv = s_cv[q];
-          uo = s_uo[q];
-        } else {
-          cv = load_child<T>(P, P.ch[c0 + q]);
-          uo = P.uoff[P.ch[c0 + q]];
+      // loads issued unconditionally (indices clamped), then selected
+      const double xv = xr[P.node_to_var[vp + min(pi, ns - 1)]];
+      double v = pi < ns ? xv : 0.0;
+      if (cached) {
+        // four children at a time: map lookups, then update-vector entries,
+        // all in flight together; the sum stays in child order
+        for (int q0 = 0; q0 < nch; q0 += 4) {
+          int rc[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int rq = child_row_nb(P, s_cv[min(q0 + u, nch - 1)], pi, ns, din);
+            rc[u] = q0 + u < nch ? rq : -1;
+          }
+          double uv[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int q = min(q0 + u, nch - 1);
+            uv[u] = P.U[s_uo[q] + (long long)r * ustride + max(rc[u] - s_cv[q].npiv, 0)];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (rc[u] >= 0) v += uv[u];
         }
-        int rc = child_row(P, cv, pi, ns, din);
-        if (rc >= 0) v += P.U[uo + (long long)r * ustride + (rc - cv.npiv)];
+      } else {
+        for (int q = 0; q < nch; q++) {
+          const ChildView<T> cv = load_child<T>(P, P.ch[c0 + q]);
+          const long long uo = P.uoff[P.ch[c0 + q]];
+          const int rc = child_row(P, cv, pi, ns, din);
+          if (rc >= 0) v += P.U[uo + (long long)r * ustride + (rc - cv.npiv)];
+        }
       }
       t[i] = v;
     }
