@@ -1128,7 +1128,10 @@ __device__ inline double wave_max(double x) {
   return r;
 }
 
-template <int NT, int R, class T>
+// BKC: the launch may hold a root front (Bunch-Kaufman tests, mode 1); the
+// instances without it carry no Bunch-Kaufman code, so that the kernel the
+// other levels run is smaller.
+template <int NT, int R, class T, bool BKC>
 __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
   constexpr int NB = kNBLarge;
   constexpr int NW = NT / kWave;
@@ -1202,7 +1205,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
     // the sequential loop.  One barrier for the choices and one for the
     // checks replace the per-pivot reductions and barriers.
     const bool spec = P.spec && mode != 2;
-    const bool bk = mode == 1;  // Bunch-Kaufman tests (the root front)
+    const bool bk = BKC && mode == 1;  // Bunch-Kaufman tests (the root front)
     if (spec) {
       T fin[NB][R];  // front columns kstart .. kstart + NB - 1 of the owned rows
 #pragma unroll
@@ -1585,7 +1588,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
       const double colmax = fmax(awik, lrest);
       int step = 0, kpiv = -1;
       bool zero_col = false, need_row = false;
-      if (mode == 1) {
+      if (BKC && mode == 1) {
         if (fmax(awkk, colmax) == 0.0 || isnan(awkk)) {
           step = 1;
           kpiv = k;
@@ -1669,7 +1672,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
         wik = (imax == k + 1) ? sh.wik[par] : sh.swik;
         const double rowmax = lane_d(row_max(lane < NW ? sh.qr[par][lane] : 0.0), 0);
         const double awii = fabs(wii);
-        if (mode == 1) {
+        if (BKC && mode == 1) {
           if (awkk >= alpha * colmax * (colmax / rowmax)) {
             step = 1;
             kpiv = k;
@@ -3444,7 +3447,7 @@ static size_t lds_front_bytes(int c, size_t esize) {
 // 512 threads with R = 1, 2 or 4 rows each.  False (use the global-memory
 // panel) when the panel's L does not fit LDS: above 2048 rows for FP32
 // fronts, above 1024 rows for FP64 fronts.
-template <class T>
+template <class T, bool BKC>
 static bool launch_panel_rr(int maxcap, dim3 grid, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
                             const DevPlan& P, const int* nodes) {
   constexpr int NT = 512;
@@ -3455,14 +3458,14 @@ static bool launch_panel_rr(int maxcap, dim3 grid, hipStream_t st, hipEvent_t e0
     // fronts of at most 256 rows: 4 waves, every one holding rows
     constexpr int NH = NT / 2;
     const uint32_t b = (uint32_t)(sizeof(T) * kNBLarge * (size_t)NH + sizeof(int) * 2 * (size_t)maxcap);
-    hipExtLaunchKernelGGL((k_panel_rr<NH, 1, T>), grid, dim3(NH), b, st, e0, e1, 0, P, nodes);
+    hipExtLaunchKernelGGL((k_panel_rr<NH, 1, T, BKC>), grid, dim3(NH), b, st, e0, e1, 0, P, nodes);
   } else if (maxcap <= NT) {
-    hipExtLaunchKernelGGL((k_panel_rr<NT, 1, T>), grid, dim3(NT), lds(1), st, e0, e1, 0, P, nodes);
+    hipExtLaunchKernelGGL((k_panel_rr<NT, 1, T, BKC>), grid, dim3(NT), lds(1), st, e0, e1, 0, P, nodes);
   } else if (maxcap <= 2 * NT) {
-    hipExtLaunchKernelGGL((k_panel_rr<NT, 2, T>), grid, dim3(NT), lds(2), st, e0, e1, 0, P, nodes);
+    hipExtLaunchKernelGGL((k_panel_rr<NT, 2, T, BKC>), grid, dim3(NT), lds(2), st, e0, e1, 0, P, nodes);
   } else if (maxcap <= 4 * NT && sizeof(T) == 4) {
     if constexpr (sizeof(T) == 4)
-      hipExtLaunchKernelGGL((k_panel_rr<NT, 4, T>), grid, dim3(NT), lds(4), st, e0, e1, 0, P,
+      hipExtLaunchKernelGGL((k_panel_rr<NT, 4, T, BKC>), grid, dim3(NT), lds(4), st, e0, e1, 0, P,
                             nodes);
   } else {
     return false;
@@ -3530,12 +3533,21 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
     }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     int maxcap_large = 0;
-    for (int q = g0; q < g1; q++) maxcap_large = std::max(maxcap_large, H.cap[H.large_nodes[q]]);
+    bool root_large = false;  // a root front (Bunch-Kaufman tests) in the level
+    for (int q = g0; q < g1; q++) {
+      maxcap_large = std::max(maxcap_large, H.cap[H.large_nodes[q]]);
+      root_large = root_large || H.mode[H.large_nodes[q]] == 1;
+    }
     for (int t = 0; t < H.large_panels[l]; t++) {
       {
         ProfScope ps_(H, st, KC_PANEL, true);
-        if (!launch_panel_rr<T>(maxcap_large, dim3(g1 - g0), st, ps_.start(), ps_.stop(), P,
-                                H.d_large_nodes.p + g0))
+        const bool rr = root_large ? launch_panel_rr<T, true>(maxcap_large, dim3(g1 - g0), st,
+                                                              ps_.start(), ps_.stop(), P,
+                                                              H.d_large_nodes.p + g0)
+                                   : launch_panel_rr<T, false>(maxcap_large, dim3(g1 - g0), st,
+                                                               ps_.start(), ps_.stop(), P,
+                                                               H.d_large_nodes.p + g0);
+        if (!rr)
           hipExtLaunchKernelGGL((k_panel<kPanelThreads, T>), dim3(g1 - g0), dim3(kPanelThreads),
                                 (uint32_t)(sizeof(int) * 2 * (size_t)maxcap_large), st,
                                 ps_.start(), ps_.stop(), 0, P, H.d_large_nodes.p + g0);
