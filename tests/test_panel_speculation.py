@@ -38,7 +38,23 @@ def _random_indefinite(n, density, zero_frac, seed):
             np.zeros(n), rng.standard_normal(n))
 
 
+def _forest():
+    """Two Poisson control KKTs of different sizes as one block-diagonal
+    matrix: two roots, the smaller one on a level that also holds large
+    non-root fronts of the larger component."""
+    parts = [W.poisson2d(N=96), W.poisson2d(N=40, seed=1)]
+    K = sp.block_diag([sp.csr_matrix((P.vals, P.cols, P.rowp), shape=(P.n, P.n)) for P in parts],
+                      format="csr")
+    K.sort_indices()
+    diag = np.concatenate([P.diag for P in parts])
+    rhs = np.concatenate([P.rhs for P in parts])
+    return (K.shape[0], K.indptr.astype(np.int32), K.indices.astype(np.int32), K.data.copy(),
+            diag, rhs, OrderingType.NESTED_DISSECTION), parts
+
+
 def _case(name):
+    if name == "forest":
+        return _forest()[0]
     if name == "poisson2d":
         P = W.poisson2d(N=96)
         return P.n, P.rowp, P.cols, P.vals, P.diag, P.rhs, OrderingType.NESTED_DISSECTION
@@ -60,7 +76,7 @@ def _run(monkeypatch, case, precision, spec):
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("name", ["poisson2d", "elasticity3d", "random_1", "random_2",
+@pytest.mark.parametrize("name", ["poisson2d", "elasticity3d", "forest", "random_1", "random_2",
                                   "random_3"])
 def test_speculative_panel_equals_sequential(monkeypatch, name, precision):
     print("case", name, precision, flush=True)
@@ -79,3 +95,25 @@ def test_speculative_panel_equals_sequential(monkeypatch, name, precision):
     K = sp.csr_matrix((vals, cols, rowp), shape=(n, n)) + sp.diags(diag)
     r = np.linalg.norm(K @ xa - rhs) / np.linalg.norm(rhs)
     assert r < (1e-8 if precision == FP64 else 1e-2)
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
+def test_forest_inertia_is_the_sum_of_its_components(monkeypatch, precision):
+    """A level holding a root front (Bunch-Kaufman tests) next to large
+    non-root fronts: the inertia of the block-diagonal matrix is the sum of
+    its blocks' inertias, each factored on its own."""
+    monkeypatch.setenv("KKT_LDL_PANEL_SPECULATION", "1")
+    case, parts = _forest()
+    ia, _, x = _run(monkeypatch, case, precision, "1")
+    ib = [0, 0]
+    for P in parts:
+        s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
+                      order=OrderingType.NESTED_DISSECTION, precision=precision)
+        s.factor(P.diag)
+        pos, neg = s.get_inertia()
+        ib[0] += pos
+        ib[1] += neg
+    assert tuple(ia) == tuple(ib)
+    n, rowp, cols, vals, diag, rhs, _ = case
+    K = sp.csr_matrix((vals, cols, rowp), shape=(n, n)) + sp.diags(diag)
+    assert np.linalg.norm(K @ x - rhs) / np.linalg.norm(rhs) < (1e-8 if precision == FP64 else 1e-2)
