@@ -40,6 +40,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -48,6 +49,7 @@
 
 #include "../../include/kkt_ldl.h"
 #include "kkt_common.hpp"
+#include "refine_policy.hpp"
 #include "symbolic.hpp"
 
 namespace kktldl {
@@ -1280,12 +1282,48 @@ __global__ void k_colnorm2_final(int nrhs, const double* part, double* out) {
   out[c] = t;
 }
 
-__global__ void k_axpy_cols(int n, double* x, const double* dx, int ld, int nrhs) {
+// x += dx over nrhs columns, keeping the previous x in xb (the refinement
+// returns to it if the step raised the residual)
+__global__ void k_axpy_cols(int n, double* x, const double* dx, double* xb, int ld, int nrhs) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t tot = (size_t)n * nrhs;
   if (i >= tot) return;
   size_t c = i / n, r = i % n;
-  x[r + c * ld] += dx[r + c * ld];
+  const double v = x[r + c * ld];
+  xb[r + c * ld] = v;
+  x[r + c * ld] = v + dx[r + c * ld];
+}
+
+// Zero and non-finite pivots of the last factorization, one wave per node:
+// out[2p] = exact zero 1x1 pivots (Cholesky: zero diagonal of L) plus 2x2
+// blocks with det = 0, out[2p+1] = pivots whose FP64 D entries are NaN or
+// infinite.  Reads the FP64 D that the inertia counts come from.
+__global__ void __launch_bounds__(kWave) k_pivot_check(DevPlan P, int* out) {
+  const int p = blockIdx.x;
+  const int npiv = P.npiv[p];
+  const long long io = P.ioff[p];
+  int nz = 0, nf = 0;
+  for (int j = threadIdx.x; j < npiv; j += kWave) {
+    const int t = P.ptype[io + j];
+    const double d = P.dv[io + j];
+    if (t == 1) {
+      nz += d == 0.0;
+      nf += !isfinite(d);
+    } else if (t == 2 && j + 1 < npiv) {
+      const double e = P.dv[io + j + 1], c = P.dv2[io + j];
+      nz += (d * e - c * c) == 0.0;
+      nf += !(isfinite(d) && isfinite(e) && isfinite(c));
+    }
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    nz += __shfl_xor(nz, o);
+    nf += __shfl_xor(nf, o);
+  }
+  if (threadIdx.x == 0) {
+    out[2 * p] = nz;
+    out[2 * p + 1] = nf;
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -1378,10 +1416,11 @@ struct Handle {
   DBuf<int> d_inv_node, d_inv_panel, d_inv2_node, d_inv2_pair;
   DBuf<double> d_pinv;
   int y_nrhs = 0;
-  DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_norm, d_dv, d_dv2;
+  DBuf<double> d_F, d_W, d_U, d_vals, d_diag, d_x, d_r, d_b, d_xb, d_norm, d_dv, d_dv2;
   int esize = 8;  // bytes per front entry: 8 (FP64) or 4 (FP32 factor)
   DBuf<int> d_vars, d_pre, d_ptype;
   bool has_diag = false;
+  bool vals_loaded = false;  // d_vals holds the values of the last kkt_ldl_factor
   bool factored = false;
   bool uploaded = false;
   int u_nrhs = 0;
@@ -1719,7 +1758,7 @@ static void upload_static(Handle& H) {
   H.d_doff.alloc(N);
   H.d_stat.alloc(N);
   H.d_need.alloc(N);
-  H.d_cnt.alloc(3 * (size_t)N);
+  H.d_cnt.alloc(5 * (size_t)N);  // [3N] inertia counters, [2N] zero / non-finite pivots
   H.d_ps.alloc(N);
   H.d_small_nodes.upload(H.small_nodes);
   H.d_lds_nodes.upload(H.lds_nodes);
@@ -1969,6 +2008,8 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       hipLaunchKernelGGL(k_panel_inv2<T>, dim3((unsigned)H.inv2_node.size()), dim3(kWave), 0, st,
                          P, H.d_inv2_node.p, H.d_inv2_pair.p);
   }
+  hipLaunchKernelGGL(k_pivot_check, dim3((unsigned)H.N), dim3(kWave), 0, st, P,
+                     H.d_cnt.p + 3 * (size_t)H.N);
   HIPCHECK(hipGetLastError());
 }
 
@@ -2004,14 +2045,21 @@ static void ensure_device(Handle& H) {
   H.uploaded = true;
 }
 
+// vals == nullptr: refactorization with the values of the last call, which
+// stay on the device (only the diagonal is new).
 static int factor_impl(Handle& H, const double* vals, const double* diag, int mem,
                        hipStream_t st) {
   auto t0 = std::chrono::steady_clock::now();
+  if (mem != KKT_LDL_HOST && mem != KKT_LDL_DEVICE) throw Error(-1, "unknown memory space");
+  if (!vals && !H.vals_loaded) throw Error(-3, "refactor called before factor");
   ensure_device(H);
-  H.d_vals.alloc(H.nnz);
   const hipMemcpyKind kind = mem == KKT_LDL_DEVICE ? hipMemcpyDeviceToDevice
                                                    : hipMemcpyHostToDevice;
-  HIPCHECK(hipMemcpyAsync(H.d_vals.p, vals, sizeof(double) * H.nnz, kind, st));
+  if (vals) {
+    H.d_vals.alloc(H.nnz);
+    HIPCHECK(hipMemcpyAsync(H.d_vals.p, vals, sizeof(double) * H.nnz, kind, st));
+    H.vals_loaded = true;
+  }
   H.has_diag = diag != nullptr;
   if (diag) {
     H.d_diag.alloc(H.n);
@@ -2051,28 +2099,34 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
   prof_collect(H);
   H.h_npiv.resize(H.N);
   H.h_ndel.resize(H.N);
-  H.h_cnt.resize(3 * (size_t)H.N);
+  H.h_cnt.resize(5 * (size_t)H.N);
   HIPCHECK(hipMemcpy(H.h_npiv.data(), H.d_npiv.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(H.h_ndel.data(), H.d_ndel.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(H.h_cnt.data(), H.d_cnt.p, sizeof(int) * 3 * H.N, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(H.h_cnt.data(), H.d_cnt.p, sizeof(int) * 5 * H.N, hipMemcpyDeviceToHost));
   // the solve launches only the panel steps that hold pivots
   const int nlev = (int)H.lvl_ptr.size() - 1;
   H.lvl_steps.assign(nlev, 0);
   for (int l = 0; l < nlev; l++)
     for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++)
       H.lvl_steps[l] = std::max(H.lvl_steps[l], (H.h_npiv[H.large_nodes[q]] + kPS - 1) / kPS);
-  int nd = 0, n2 = 0;
+  int nd = 0, n2 = 0, nz = 0, nf = 0;
   long long stored = 0;
   for (int k = 0; k < H.N; k++) {
     nd += H.h_ndel[k];
     n2 += H.h_cnt[3 * k + 2];
+    nz += H.h_cnt[3 * (size_t)H.N + 2 * k];
+    nf += H.h_cnt[3 * (size_t)H.N + 2 * k + 1];
   }
   H.stats.num_delayed = nd;
   H.stats.num_2x2 = n2;
+  H.stats.num_zero_pivots = nz;
+  H.stats.num_nonfinite_pivots = nf;
   H.stats.stored_factor = stored;
   H.factored = true;
   H.stats.t_factor_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (nf > 0) return KKT_LDL_WARN_NONFINITE;
+  if (nz > 0) return KKT_LDL_WARN_SINGULAR;
   return 0;
 }
 
@@ -2193,6 +2247,7 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
   if (refine) {
     H.d_b.alloc(tot);
     H.d_r.alloc(tot);
+    H.d_xb.alloc(tot);
     H.d_norm.alloc(2 * (size_t)nrhs + (size_t)kNormBlocks * nrhs);
     HIPCHECK(hipMemcpyAsync(H.d_b.p, dx, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
     std::vector<double> bn(nrhs), rn(nrhs);
@@ -2207,7 +2262,7 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
     HIPCHECK(hipMemcpyAsync(bn.data(), H.d_norm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     run_solve_levels(H, dx, ldx, nrhs, st);
     int it = 0;
-    double worst = 0.0;
+    double best = std::numeric_limits<double>::infinity();
     const dim3 rg((unsigned)((H.n * 16 + kTPB - 1) / kTPB));
     for (;; it++) {
       {
@@ -2219,17 +2274,25 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
       colnorm2(H.d_r.p, H.d_norm.p);
       HIPCHECK(hipMemcpyAsync(rn.data(), H.d_norm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
-      worst = 0.0;
-      for (int c = 0; c < nrhs; c++)
-        worst = std::max(worst, bn[c] > 0 ? std::sqrt(rn[c] / bn[c]) : std::sqrt(rn[c]));
-      if (worst <= H.opt.refine_tol || it >= H.opt.max_refine) break;
+      const double w = refine_worst(bn.data(), rn.data(), nrhs);
+      const int act = refine_next(it, w, best, H.opt.refine_tol, H.opt.max_refine);
+      if (act == REFINE_UNDO_LAST) {
+        // the last correction did not lower the residual: back to the
+        // iterate before it, whose residual is `best`
+        HIPCHECK(hipMemcpyAsync(dx, H.d_xb.p, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
+        it--;
+        break;
+      }
+      best = w;
+      if (act == REFINE_DONE) break;
       run_solve_levels(H, H.d_r.p, ldx, nrhs, st);
       const size_t tb = ((size_t)H.n * nrhs + kTPB - 1) / kTPB;
-      hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)tb), dim3(kTPB), 0, st, H.n, dx, H.d_r.p, ldx, nrhs);
+      hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)tb), dim3(kTPB), 0, st, H.n, dx, H.d_r.p,
+                         H.d_xb.p, ldx, nrhs);
     }
     H.stats.refine_steps = it;
-    H.stats.residual = worst;
-    if (worst > H.opt.refine_tol) status = KKT_LDL_WARN_NOT_CONVERGED;
+    H.stats.residual = best;
+    if (!(best <= H.opt.refine_tol)) status = KKT_LDL_WARN_NOT_CONVERGED;
   } else {
     run_solve_levels(H, dx, ldx, nrhs, st);
   }
@@ -2266,6 +2329,13 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   }
   std::vector<int> perm;
   const int* pp = nullptr;
+  if (H.opt.order == KKT_LDL_ORDER_MULTI_COLOR)
+    throw Error(-1, "MULTI_COLOR ordering is not supported");
+  if (H.opt.order != KKT_LDL_ORDER_ND && H.opt.order != KKT_LDL_ORDER_AMD &&
+      H.opt.order != KKT_LDL_ORDER_NATURAL && H.opt.order != KKT_LDL_ORDER_GIVEN)
+    throw Error(-1, "unknown ordering");
+  if (H.opt.solver_type != KKT_LDL_LDL && H.opt.solver_type != KKT_LDL_CHOLESKY)
+    throw Error(-1, "unknown solver type");
   if (H.opt.order == KKT_LDL_ORDER_AMD) {
     perm.resize(n);
     amd_ordering(n, rowp, cols, perm.data());
@@ -2305,6 +2375,25 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   }
   H.stats.flops_factor = fl;
   H.stats.nnz_l = nz;
+  // critical path of the assembly tree in pivot columns, and the sum over
+  // levels of the largest panel count, which bounds the panel launches of
+  // the level schedule (children are numbered before their parents)
+  {
+    std::vector<long long> path(H.N, 0);
+    long long cp = 0;
+    for (int k = 0; k < H.N; k++) {
+      long long below = 0;
+      for (int q = H.S.children_ptr[k]; q < H.S.children_ptr[k + 1]; q++)
+        below = std::max(below, path[H.S.children[q]]);
+      path[k] = below + H.ns[k];
+      cp = std::max(cp, path[k]);
+    }
+    H.stats.crit_path_pivots = cp;
+    const int nlev = (int)H.lvl_ptr.size() - 1;
+    long long lp = 0;
+    for (int l = 0; l < nlev; l++) lp += H.large_panels[l];
+    H.stats.level_panel_sum = lp;
+  }
   H.stats.t_analyze_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -2368,6 +2457,23 @@ int kkt_ldl_factor(kkt_ldl_t h, const double* vals, const double* diag, int mem,
   });
 }
 
+int kkt_ldl_refactor(kkt_ldl_t h, const double* diag, int mem, void* stream) {
+  return guarded([&] {
+    if (!h) throw Error(-1, "null argument");
+    return kktldl::factor_impl(h->h, nullptr, diag, mem, (hipStream_t)stream);
+  });
+}
+
+int kkt_ldl_set_pivot_threshold(kkt_ldl_t h, double ustab, double pivot_tol) {
+  return guarded([&] {
+    if (!h) throw Error(-1, "null argument");
+    if (std::isnan(ustab) || std::isnan(pivot_tol)) throw Error(-1, "NaN pivot threshold");
+    if (ustab >= 0.0) h->h.opt.ustab = ustab;
+    if (pivot_tol >= 0.0) h->h.opt.pivot_tol = pivot_tol;
+    return 0;
+  });
+}
+
 int kkt_ldl_solve(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem, void* stream) {
   return guarded([&] {
     if (!h || !x) throw Error(-1, "null argument");
@@ -2396,7 +2502,11 @@ int kkt_ldl_inertia(kkt_ldl_t h, int* npos, int* nneg, int* nzero) {
       }
     *npos = a;
     *nneg = b;
-    if (nzero) *nzero = 0;
+    if (nzero) {
+      int z = 0;
+      for (int k = 0; k < H.N; k++) z += H.h_cnt[3 * (size_t)H.N + 2 * k];
+      *nzero = z;
+    }
     return 0;
   });
 }

@@ -753,7 +753,7 @@ void analyze(int n, const int* rowp, const int* cols, const int* perm_in,
   for (int s = 0; s < ns; s++) node_of_sn[s] = new_id[top(s)];
   S.num_nodes = N;
   S.node_ptr.assign(N + 1, 0);
-  for (int s = 0; s < ns; s++) S.node_ptr[node_of_sn[s] + 1] += width[s] * 0 + (sn_ptr[s + 1] - sn_ptr[s]);
+  for (int s = 0; s < ns; s++) S.node_ptr[node_of_sn[s] + 1] += sn_ptr[s + 1] - sn_ptr[s];
   for (int k = 0; k < N; k++) S.node_ptr[k + 1] += S.node_ptr[k];
   S.node_to_var.assign(n, 0);
   vector<int> node_of_var(n), ncc(N), fill(S.node_ptr.begin(), S.node_ptr.end() - 1);

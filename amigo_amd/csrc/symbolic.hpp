@@ -9,8 +9,6 @@
 
 namespace kktldl {
 
-enum Ordering { ORDER_NATURAL = 0, ORDER_AMD = 1, ORDER_ND = 2, ORDER_GIVEN = 3 };
-
 struct SymbolicResult {
   int n = 0;
   bool permuted = false;

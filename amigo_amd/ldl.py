@@ -22,16 +22,49 @@ LIB_PATH = os.path.join(_HERE, "libkktldl.so")
 
 
 class OrderingType:
-    """Orderings (include/ordering_utils.h:22).  DEFAULT is AMD, as in a
-    reference build without METIS (amigo/amigo.cpp:329-333);
-    NESTED_DISSECTION is this engine's own nested dissection."""
+    """Orderings with the reference's numbering (include/ordering_utils.h:21,
+    bound at amigo/amigo.cpp:324-334), so that the reference's own enum values
+    (``int(amigo.OrderingType.X)``) select the same ordering here.  DEFAULT is
+    AMD, as in a reference build without METIS; NESTED_DISSECTION is this
+    engine's own nested dissection (METIS is not available).  MULTI_COLOR is
+    rejected (the reference reads an uninitialised permutation for it).
+    GIVEN (a caller's permutation) is this interface's addition."""
 
-    NATURAL = 0
+    NESTED_DISSECTION = 0
     AMD = 1
-    NESTED_DISSECTION = 2
-    GIVEN = 3
+    MULTI_COLOR = 2
+    NATURAL = 3
+    GIVEN = 4
     DEFAULT = 1
-    MULTI_COLOR = -1  # reads an uninitialised permutation in the reference
+
+
+_ORDER_NAMES = {"NESTED_DISSECTION": OrderingType.NESTED_DISSECTION,
+                "AMD": OrderingType.AMD, "MULTI_COLOR": OrderingType.MULTI_COLOR,
+                "NATURAL": OrderingType.NATURAL, "DEFAULT": OrderingType.DEFAULT,
+                "GIVEN": OrderingType.GIVEN}
+
+
+def ordering_code(order):
+    """ABI code of an ordering given as this module's int, the reference's
+    pybind enum (matched by name: its ``.name`` attribute or its ``str``
+    'OrderingType.X'), or a name string.  Raises ValueError for unknown
+    values and for MULTI_COLOR."""
+    name = getattr(order, "name", None)
+    if name is None and not isinstance(order, (int, np.integer)):
+        name = str(order)
+    if name is not None and name.startswith("OrderingType."):
+        name = name.split(".", 1)[1]
+    if name is not None:
+        if name not in _ORDER_NAMES:
+            raise ValueError(f"unknown ordering {order!r}")
+        code = _ORDER_NAMES[name]
+    else:
+        code = int(order)
+        if code not in _ORDER_NAMES.values():
+            raise ValueError(f"unknown ordering {order!r}")
+    if code == OrderingType.MULTI_COLOR:
+        raise ValueError("MULTI_COLOR ordering is not supported")
+    return code
 
 
 class SolverType:
@@ -45,6 +78,7 @@ HOST = 0
 DEVICE = 1
 WARN_SINGULAR = 1
 WARN_NOT_CONVERGED = 2
+WARN_NONFINITE = 3
 
 
 class Options(ctypes.Structure):
@@ -80,6 +114,10 @@ class Stats(ctypes.Structure):
         ("t_factor_ms", ctypes.c_double),
         ("t_solve_ms", ctypes.c_double),
         ("front_bytes", ctypes.c_longlong),
+        ("num_zero_pivots", ctypes.c_int),
+        ("num_nonfinite_pivots", ctypes.c_int),
+        ("crit_path_pivots", ctypes.c_longlong),
+        ("level_panel_sum", ctypes.c_longlong),
     ]
 
     def as_dict(self):
@@ -89,6 +127,7 @@ class Stats(ctypes.Structure):
 # Every symbol include/kkt_ldl.h declares.
 EXPORTS = (
     "kkt_ldl_default_options", "kkt_ldl_create", "kkt_ldl_factor",
+    "kkt_ldl_refactor", "kkt_ldl_set_pivot_threshold",
     "kkt_ldl_solve", "kkt_ldl_solve_refined", "kkt_ldl_inertia",
     "kkt_ldl_get_perm", "kkt_ldl_get_sizes", "kkt_ldl_get_supernodes",
     "kkt_ldl_get_node_info", "kkt_ldl_get_stats", "kkt_ldl_last_times",
@@ -124,6 +163,8 @@ def load_library(path=LIB_PATH):
     lib.kkt_ldl_create.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.POINTER(Options),
                                    ctypes.POINTER(_vp)]
     lib.kkt_ldl_factor.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+    lib.kkt_ldl_refactor.argtypes = [_vp, _vp, ctypes.c_int, _vp]
+    lib.kkt_ldl_set_pivot_threshold.argtypes = [_vp, ctypes.c_double, ctypes.c_double]
     lib.kkt_ldl_solve.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]
     lib.kkt_ldl_solve_refined.argtypes = lib.kkt_ldl_solve.argtypes
     lib.kkt_ldl_inertia.argtypes = [_vp, _i32p, _i32p, _i32p]
@@ -253,8 +294,7 @@ class SparseLDL:
     def __init__(self, mat, solver_type=SolverType.LDL, ustab=0.01, pivot_tol=1e-14,
                  delay_growth=2.0, order=OrderingType.NATURAL, precision=FP64,
                  refine_tol=1e-10, max_refine=10, perm=None):
-        if order == OrderingType.MULTI_COLOR:
-            raise ValueError("MULTI_COLOR ordering is not supported")
+        order = ordering_code(order)
         self.lib = load_library()
         self.mat = mat
         nrows, ncols, nnz, rowp, cols = mat.get_nonzero_structure()
@@ -266,7 +306,7 @@ class SparseLDL:
         self._cols = np.ascontiguousarray(cols, dtype=np.int32)
         opts = Options()
         self.lib.kkt_ldl_default_options(ctypes.byref(opts))
-        opts.order = int(order)
+        opts.order = order
         opts.solver_type = int(solver_type)
         opts.ustab = float(ustab)
         opts.pivot_tol = float(pivot_tol)
@@ -293,6 +333,11 @@ class SparseLDL:
 
     # -- reference interface -------------------------------------------------
     def factor(self, diagonal=None):
+        """Numeric factorization of the matrix's current values plus
+        ``diagonal``.  Returns 0, or a warning: WARN_SINGULAR (zero pivots,
+        counted positive in the inertia as the reference does) or
+        WARN_NONFINITE (NaN / infinite pivots).  Raises RuntimeError on
+        errors."""
         vals = self.mat.get_data()
         d = None
         if diagonal is not None:
@@ -316,6 +361,39 @@ class SparseLDL:
                                          d.ctypes.data_as(_vp) if d is not None else None,
                                          HOST, None)
         return _check(st, "kkt_ldl_factor")
+
+    def refactor(self, diagonal=None):
+        """Refactorization with a new diagonal and the matrix values of the
+        last ``factor`` call (kept on the device; they are not re-read).  For
+        inertia-correction retries, which change only the diagonal."""
+        d = None
+        if diagonal is not None:
+            d = diagonal.get_array() if hasattr(diagonal, "get_array") else diagonal
+        if _is_torch(d):
+            import torch
+            d = d.to(torch.float64).contiguous()
+            self._keep_diag = d
+            st = self.lib.kkt_ldl_refactor(self.h, _vp(d.data_ptr()), DEVICE,
+                                           _current_stream())
+        else:
+            d = None if d is None else np.ascontiguousarray(d, dtype=np.float64)
+            st = self.lib.kkt_ldl_refactor(self.h, d.ctypes.data_as(_vp) if d is not None
+                                           else None, HOST, None)
+        return _check(st, "kkt_ldl_refactor")
+
+    def set_pivot_threshold(self, ustab=-1.0, pivot_tol=-1.0):
+        """Threshold-pivot stability factor and absolute pivot tolerance for
+        the next factorizations (negative: unchanged)."""
+        _check(self.lib.kkt_ldl_set_pivot_threshold(self.h, float(ustab), float(pivot_tol)),
+               "kkt_ldl_set_pivot_threshold")
+
+    def inertia_and_zero(self):
+        """(npos, nneg, nzero): nzero counts the zero pivots, which are
+        included in npos."""
+        a, b, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.kkt_ldl_inertia(self.h, ctypes.byref(a), ctypes.byref(b),
+                                        ctypes.byref(z)), "kkt_ldl_inertia")
+        return a.value, b.value, z.value
 
     def solve(self, x, refine=False):
         """x <- K^{-1} x in place: a Vector, a float64 ndarray (1-D, or 2-D

@@ -23,19 +23,30 @@ extern "C" {
 
 typedef struct kkt_ldl_handle* kkt_ldl_t;
 
-/* Orderings (include/ordering_utils.h:22 OrderingType). */
-enum { KKT_LDL_ORDER_NATURAL = 0, KKT_LDL_ORDER_AMD = 1, KKT_LDL_ORDER_ND = 2,
-       KKT_LDL_ORDER_GIVEN = 3 };
+/* Orderings, numbered as the reference's OrderingType (include/
+ * ordering_utils.h:21: NESTED_DISSECTION, AMD, MULTI_COLOR, NATURAL), so that
+ * int(amigo.OrderingType.X) can be passed unchanged.  MULTI_COLOR is
+ * rejected (the reference reads an uninitialised permutation for it,
+ * ordering_utils.h:280-291); GIVEN (a caller's permutation) is this ABI's
+ * own addition.  Unknown values are rejected. */
+enum { KKT_LDL_ORDER_ND = 0, KKT_LDL_ORDER_AMD = 1, KKT_LDL_ORDER_MULTI_COLOR = 2,
+       KKT_LDL_ORDER_NATURAL = 3, KKT_LDL_ORDER_GIVEN = 4 };
 /* Factorization kinds (include/sparse_ldl.h:23 SolverType). */
 enum { KKT_LDL_LDL = 0, KKT_LDL_CHOLESKY = 1 };
 /* Arithmetic of the stored factor. */
 enum { KKT_LDL_FP64 = 0, KKT_LDL_FP32_REFINE = 1 };
 /* Memory space of caller buffers. */
 enum { KKT_LDL_HOST = 0, KKT_LDL_DEVICE = 1 };
-/* Positive status codes (warnings).  KKT_LDL_WARN_SINGULAR is reserved: a
- * singular matrix factors without error, as in the reference (zero pivots
- * count positive in the inertia). */
-enum { KKT_LDL_WARN_SINGULAR = 1, KKT_LDL_WARN_NOT_CONVERGED = 2 };
+/* Positive status codes (warnings).  A singular matrix factors to the end,
+ * as in the reference, with zero pivots counted positive in the inertia
+ * (include/sparse_ldl.h:1958); kkt_ldl_factor then returns
+ * KKT_LDL_WARN_SINGULAR and kkt_ldl_inertia reports the zero pivots in
+ * nzero.  A NaN or infinite pivot (non-finite values in K or the diagonal,
+ * or an indefinite matrix in Cholesky mode) returns KKT_LDL_WARN_NONFINITE.
+ * kkt_ldl_solve_refined returns KKT_LDL_WARN_NOT_CONVERGED when the residual
+ * target is not met, including a non-finite residual. */
+enum { KKT_LDL_WARN_SINGULAR = 1, KKT_LDL_WARN_NOT_CONVERGED = 2,
+       KKT_LDL_WARN_NONFINITE = 3 };
 
 typedef struct {
   int order;            /* KKT_LDL_ORDER_*                                  */
@@ -61,6 +72,12 @@ typedef struct {
   double residual;          /* last relative residual ||Kx-b||/||b||        */
   double t_analyze_ms, t_factor_ms, t_solve_ms;
   long long front_bytes;    /* device arena holding every front             */
+  int num_zero_pivots;      /* last factor: exact zero 1x1 pivots / 2x2 dets */
+  int num_nonfinite_pivots; /* last factor: NaN or infinite pivots          */
+  long long crit_path_pivots; /* largest sum of supernode columns on a
+                               * leaf-to-root path of the assembly tree     */
+  long long level_panel_sum;  /* sum over tree levels of the largest number
+                               * of 16-column panels of a front of the level */
 } kkt_ldl_stats;
 
 /* Defaults matching AmigoSolver (amigo/algorithm/solvers/amigo_solver.py:15-25). */
@@ -79,6 +96,23 @@ int kkt_ldl_create(int n, const int* rowp, const int* cols,
 int kkt_ldl_factor(kkt_ldl_t h, const double* vals, const double* diag, int mem,
                    void* stream);
 
+/* Numeric refactorization with a new diagonal only: K's values are those
+ * of the last kkt_ldl_factor call, kept on the device, so only diag[n] is
+ * read (mem as in kkt_ldl_factor).  For the inertia-correction retries, which
+ * change only the diagonal perturbation (amigo/algorithm/solvers/
+ * inertia_correction.py:280-317, 324-383); the reference re-reads the whole
+ * matrix on each of them (include/sparse_ldl.h:136-158).  Same status codes
+ * as kkt_ldl_factor; -3 before any kkt_ldl_factor. */
+int kkt_ldl_refactor(kkt_ldl_t h, const double* diag, int mem, void* stream);
+
+/* Threshold-pivot stability factor (ustab) and absolute pivot tolerance
+ * used from the next factorization on; a negative value leaves that setting
+ * unchanged.  The reference's AmigoSolver accepts set_pivot_tolerance and
+ * ignores it (amigo/algorithm/solvers/amigo_solver.py:63-64); MUMPS maps the
+ * IPM's pivtol to its relative threshold CNTL(1) (mumps_solver.py:323-325),
+ * which is the role ustab plays here. */
+int kkt_ldl_set_pivot_threshold(kkt_ldl_t h, double ustab, double pivot_tol);
+
 /* x <- (K + diag d)^{-1} x for nrhs columns of length n with leading
  * dimension ldx (column major, FP64).  Replaces SparseLDL.solve -- include/
  * sparse_ldl.h:169-197, amigo/amigo.cpp:665-727. */
@@ -89,13 +123,17 @@ int kkt_ldl_solve(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem,
  * K + diag until ||Kx-b||/||b|| <= refine_tol (or max_refine steps).  No
  * counterpart in the reference (it never refines, amigo/algorithm/solvers/
  * linear_solver.py:5-35); used by the FP32 factor and by the IPM drop-in.
- * Returns KKT_LDL_WARN_NOT_CONVERGED if the target is not met. */
+ * The refinement stops when the worst column's residual no longer decreases
+ * and then returns the best iterate.  Returns KKT_LDL_WARN_NOT_CONVERGED if
+ * the target is not met (a NaN or infinite residual counts as not met). */
 int kkt_ldl_solve_refined(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem,
                           void* stream);
 
 /* Inertia from D of the last factorization; zero pivots count positive as in
  * the reference.  Replaces SparseLDL.get_inertia -- include/sparse_ldl.h:
- * 205-213, 1935-1987, amigo/amigo.cpp:728-731.  nzero may be NULL. */
+ * 205-213, 1935-1987, amigo/amigo.cpp:728-731.  nzero (may be NULL) is the
+ * number of zero pivots, already counted in npos; the reference does not
+ * report it. */
 int kkt_ldl_inertia(kkt_ldl_t h, int* npos, int* nneg, int* nzero);
 
 /* Fill-reducing permutation (perm[new] = old, iperm[old] = new); identity for

@@ -36,6 +36,14 @@ def golden():
     return golden_cases()
 
 
+def engine_order(c):
+    """Engine ordering code of a golden fixture.  The fixtures record the
+    oracle's numbering (0 natural, 1 AMD), the engine takes the reference's
+    OrderingType numbering (ND 0, AMD 1, MULTI_COLOR 2, NATURAL 3)."""
+    from amigo_amd.ldl import OrderingType
+    return {0: OrderingType.NATURAL, 1: OrderingType.AMD}[int(c["order"])]
+
+
 def has_gpu():
     try:
         import torch

@@ -73,10 +73,10 @@ def test_create_rejects_bad_patterns(rowp, cols, msg):
 
 
 def test_given_order_requires_a_permutation():
-    lib, st, h = _create(2, [0, 2, 4], [0, 1, 0, 1], order=3)
+    lib, st, h = _create(2, [0, 2, 4], [0, 1, 0, 1], order=L.OrderingType.GIVEN)
     assert st < 0 and "perm" in lib.kkt_ldl_last_error().decode()
     bad = np.array([0, 0], np.int32)
-    lib, st, h = _create(2, [0, 2, 4], [0, 1, 0, 1], order=3,
+    lib, st, h = _create(2, [0, 2, 4], [0, 1, 0, 1], order=L.OrderingType.GIVEN,
                          perm=bad.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     assert st < 0 and "permutation" in lib.kkt_ldl_last_error().decode()
 
@@ -100,3 +100,101 @@ def test_factor_without_gpu_fails_loudly():
     assert st < 0
     assert "HIP" in lib.kkt_ldl_last_error().decode()
     lib.kkt_ldl_destroy(h)
+
+
+class _RefEnum:
+    """Stand-in for a value of the reference's pybind OrderingType: an int
+    with a ``name`` (amigo/amigo.cpp:324-334)."""
+
+    def __init__(self, name, value):
+        self.name, self.value = name, value
+
+    def __int__(self):
+        return self.value
+
+    def __str__(self):
+        return f"OrderingType.{self.name}"
+
+
+def test_ordering_codes_follow_the_reference_enum():
+    """include/ordering_utils.h:21: NESTED_DISSECTION=0, AMD=1, MULTI_COLOR=2,
+    NATURAL=3.  Each reference value selects its own ordering."""
+    assert (L.OrderingType.NESTED_DISSECTION, L.OrderingType.AMD, L.OrderingType.MULTI_COLOR,
+            L.OrderingType.NATURAL) == (0, 1, 2, 3)
+    from amigo_amd import workloads as W
+    P = W.poisson2d(N=12)
+    mat = L.CSRMat(P.n, P.n, P.rowp, P.cols, P.vals)
+    perms = {}
+    for name, value in (("NESTED_DISSECTION", 0), ("AMD", 1), ("NATURAL", 3)):
+        for order in (value, _RefEnum(name, value), name, f"OrderingType.{name}"):
+            perms.setdefault(name, []).append(L.SparseLDL(mat, order=order).get_perm()[0])
+    for name, ps in perms.items():
+        assert all(np.array_equal(ps[0], q) for q in ps), name
+    assert np.array_equal(perms["NATURAL"][0], np.arange(P.n))
+    assert not np.array_equal(perms["AMD"][0], perms["NATURAL"][0])
+    assert not np.array_equal(perms["NESTED_DISSECTION"][0], perms["AMD"][0])
+    assert sorted(perms["NESTED_DISSECTION"][0]) == list(range(P.n))
+    assert L.SparseLDL(mat, order=_RefEnum("DEFAULT", 1)).get_perm()[0].tolist() == \
+        perms["AMD"][0].tolist()
+    for bad in (2, _RefEnum("MULTI_COLOR", 2), 5, -1, "SPECTRAL"):
+        with pytest.raises(ValueError):
+            L.SparseLDL(mat, order=bad)
+
+
+def test_abi_rejects_unknown_orderings():
+    lib, st, h = _create(2, [0, 2, 4], [0, 1, 0, 1], order=L.OrderingType.MULTI_COLOR)
+    assert st < 0 and "MULTI_COLOR" in lib.kkt_ldl_last_error().decode()
+    lib, st, h = _create(2, [0, 2, 4], [0, 1, 0, 1], order=7)
+    assert st < 0 and "unknown ordering" in lib.kkt_ldl_last_error().decode()
+
+
+def test_refactor_and_pivot_threshold_arguments():
+    lib, st, h = _create(2, [0, 2, 4], [0, 1, 0, 1])
+    assert st == 0
+    d = np.ones(2)
+    # no values loaded yet: the refactorization has nothing to reuse
+    assert lib.kkt_ldl_refactor(h, d.ctypes.data_as(ctypes.c_void_p), 0, None) == -3
+    assert "before factor" in lib.kkt_ldl_last_error().decode()
+    assert lib.kkt_ldl_set_pivot_threshold(h, 0.1, -1.0) == 0
+    assert lib.kkt_ldl_set_pivot_threshold(h, float("nan"), -1.0) < 0
+    lib.kkt_ldl_destroy(h)
+
+
+def test_refinement_policy(tmp_path):
+    """The refinement's host decisions (amigo_amd/csrc/refine_policy.hpp),
+    compiled on their own: a NaN or infinite residual never reads as
+    converged, a step that raises the residual is taken back."""
+    import subprocess
+    drv = tmp_path / "drv.cpp"
+    drv.write_text(r'''
+#include <cstdio>
+#include <cmath>
+#include "refine_policy.hpp"
+using namespace kktldl;
+int main() {
+  const double nan = std::nan(""), inf = INFINITY;
+  double b[3] = {4.0, 1.0, 0.0}, r[3];
+  r[0] = 4e-20; r[1] = 1e-22; r[2] = 0.0;  printf("%.3e\n", refine_worst(b, r, 3));
+  r[1] = nan;                               printf("%.3e\n", refine_worst(b, r, 3));
+  r[1] = 1e-22; r[0] = nan;                 printf("%.3e\n", refine_worst(b, r, 3));
+  r[0] = inf;                               printf("%.3e\n", refine_worst(b, r, 3));
+  r[0] = 4e-20; r[2] = 9.0;                 printf("%.3e\n", refine_worst(b, r, 3));
+  printf("%d %d %d %d %d %d %d\n",
+         refine_next(0, 1e-12, inf, 1e-10, 10),   // converged
+         refine_next(0, 1e-6, inf, 1e-10, 10),    // go on
+         refine_next(0, inf, inf, 1e-10, 10),     // non-finite plain solve: stop
+         refine_next(2, 1e-8, 1e-9, 1e-10, 10),   // residual grew: undo
+         refine_next(2, 1e-9, 1e-9, 1e-10, 10),   // no progress: undo
+         refine_next(1, inf, 1e-6, 1e-10, 10),    // NaN after a step: undo
+         refine_next(10, 1e-9, 1e-8, 1e-10, 10)); // step limit
+  return 0;
+}
+''')
+    exe = tmp_path / "drv"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "amigo_amd", "csrc"),
+                    str(drv), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    assert float(out[0]) == pytest.approx(1e-10)
+    assert out[1] == "inf" and out[2] == "inf" and out[3] == "inf"
+    assert float(out[4]) == pytest.approx(3.0)
+    assert out[5].split() == ["0", "1", "0", "2", "2", "2", "0"]

@@ -15,7 +15,7 @@ from amigo_amd import workloads as W
 from amigo_amd.ldl import CSRMat, OrderingType, SolverType, SparseLDL, Vector
 from oracle.pyoracle import AMD, OracleLDL
 
-from .conftest import golden_cases
+from .conftest import engine_order, golden_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +26,7 @@ def _engine(c, **kw):
     n = int(c["n"])
     return SparseLDL(CSRMat(n, n, c["rowp"], c["cols"], c["vals"]),
                      solver_type=int(c["solver_type"]), ustab=float(c["ustab"]),
-                     pivot_tol=float(c["pivot_tol"]), order=int(c["order"]), **kw)
+                     pivot_tol=float(c["pivot_tol"]), order=engine_order(c), **kw)
 
 
 def _K(c):

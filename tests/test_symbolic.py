@@ -10,7 +10,7 @@ from amigo_amd import workloads as W
 from amigo_amd.ldl import CSRMat, OrderingType, SolverType, SparseLDL
 from oracle.pyoracle import GIVEN, OracleLDL
 
-from .conftest import golden_cases
+from .conftest import engine_order, golden_cases
 
 CASES = golden_cases()
 
@@ -20,7 +20,7 @@ def test_engine_symbolic_bit_exact(c):
     n = int(c["n"])
     s = SparseLDL(CSRMat(n, n, c["rowp"], c["cols"], c["vals"]),
                   solver_type=int(c["solver_type"]), ustab=float(c["ustab"]),
-                  order=int(c["order"]))
+                  order=engine_order(c))
     perm, iperm = s.get_perm()
     assert np.array_equal(perm, c["perm"])
     assert np.array_equal(iperm, c["iperm"])
