@@ -264,6 +264,7 @@ template <int NT, class T>
 __device__ inline void sym_swap(T* F, int ld, int m, int* vars, int* pre,
                                 int a, int b) {
   const int tid = threadIdx.x;
+  if (a < 0 || b >= m || a >= b) return;  // positions outside the front: no swap
   if (tid == 0) {
     int t = vars[a];
     vars[a] = vars[b];
@@ -334,8 +335,12 @@ __device__ void panel_factor(T* F, int ld, int m, int* vars, int* pre,
     int imax = 0;
     double awik = 0.0;
     if (k + 1 < nc) {
-      imax = lidx;
-      awik = lbest;  // |W[imax, kw]|
+      // lidx keeps its sentinel only when every candidate entry is NaN (no
+      // comparison succeeds): take the next candidate, so that the row
+      // search and the swaps stay inside the front
+      const bool found = lidx < nc;
+      imax = found ? lidx : k + 1;
+      awik = found ? lbest : __builtin_nan("");  // |W[imax, kw]|
     }
     const double colmax = fmax(awik, lrest);
     int step = 0, kpiv = -1;

@@ -77,6 +77,10 @@ __global__ void k_prepare(DevPlan P, const int* nodes, int count) {
   const int m = P.ns[p] + din + P.cs[p];
   P.din[p] = din;
   P.m[p] = m;
+  // defined until the node's factor kernel writes them (a node that fails
+  // below is skipped by those kernels)
+  P.npiv[p] = 0;
+  P.ndel[p] = 0;
   if (fail || m > P.cap[p]) {
     P.stat[p] = 1;
     P.need[p] = fail ? -1 : m;
@@ -1300,7 +1304,8 @@ __global__ void k_axpy_cols(int n, double* x, const double* dx, double* xb, int 
 // infinite.  Reads the FP64 D that the inertia counts come from.
 __global__ void __launch_bounds__(kWave) k_pivot_check(DevPlan P, int* out) {
   const int p = blockIdx.x;
-  const int npiv = P.npiv[p];
+  // pivots lie in the node's slot; a node that failed holds none
+  const int npiv = P.stat[p] ? 0 : min(max(P.npiv[p], 0), P.cap[p]);
   const long long io = P.ioff[p];
   int nz = 0, nf = 0;
   for (int j = threadIdx.x; j < npiv; j += kWave) {
@@ -2008,8 +2013,6 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       hipLaunchKernelGGL(k_panel_inv2<T>, dim3((unsigned)H.inv2_node.size()), dim3(kWave), 0, st,
                          P, H.d_inv2_node.p, H.d_inv2_pair.p);
   }
-  hipLaunchKernelGGL(k_pivot_check, dim3((unsigned)H.N), dim3(kWave), 0, st, P,
-                     H.d_cnt.p + 3 * (size_t)H.N);
   HIPCHECK(hipGetLastError());
 }
 
@@ -2094,6 +2097,14 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
     build_plan(H, slack);
     upload_static(H);
     H.stats.replans++;
+  }
+  // zero and non-finite pivots of the (successful) factorization
+  {
+    DevPlan P = dev_plan(H);
+    hipLaunchKernelGGL(k_pivot_check, dim3((unsigned)H.N), dim3(kWave), 0, st, P,
+                       H.d_cnt.p + 3 * (size_t)H.N);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
   }
   HIPCHECK(hipEventElapsedTime(&H.last_factor_ms, H.ev0, H.ev1));
   prof_collect(H);
@@ -2355,7 +2366,17 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
     pp = perm.data();
   }
   analyze(n, rowp, cols, pp, H.S);
+  if (H.opt.order == KKT_LDL_ORDER_ND && H.opt.vals_hint) {
+    // predicted 2x2 partners adjacent inside every fundamental supernode
+    // (same fill and supernodes, new order inside them)
+    std::vector<int> paired(n);
+    pair_partners(n, rowp, cols, H.opt.vals_hint, H.opt.ustab, H.S, paired.data());
+    SymbolicResult S2;
+    analyze(n, rowp, cols, paired.data(), S2);
+    H.S = std::move(S2);
+  }
   H.opt.perm = nullptr;
+  H.opt.vals_hint = nullptr;
   build_maps(H);
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));
@@ -2435,6 +2456,7 @@ void kkt_ldl_default_options(kkt_ldl_options* o) {
   o->refine_tol = 1e-10;
   o->max_refine = 10;
   o->perm = nullptr;
+  o->vals_hint = nullptr;
 }
 
 int kkt_ldl_create(int n, const int* rowp, const int* cols,

@@ -548,8 +548,11 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
       int imax = 0;
       double awik = 0.0;
       if (k + 1 < nc) {
-        imax = lidx;
-        awik = lbest;
+        // lidx keeps its sentinel only when every candidate entry is NaN:
+        // take the next candidate, so that every index stays in the front
+        const bool found = lidx < nc;
+        imax = found ? lidx : k + 1;
+        awik = found ? lbest : __builtin_nan("");
       }
       const double colmax = fmax(awik, lrest);
       int step = 0, kpiv = -1;

@@ -351,206 +351,6 @@ void amd_ordering(int n, const int* rowp, const int* cols, int* perm) {
 }
 
 // ===========================================================================
-// Nested dissection (this build's own; the reference calls METIS_NodeND).
-// ===========================================================================
-namespace {
-
-struct NDWork {
-  int n;
-  const int* rowp;
-  const int* cols;
-  vector<int> mark, dist, queue, part;
-  int stamp = 0;
-};
-
-// BFS restricted to vertices with part == id; returns the last vertex
-// reached and fills dist (levels) for reached vertices.
-int nd_bfs(NDWork& W, int src, int id, int& nlev, vector<int>& order) {
-  order.clear();
-  W.stamp++;
-  order.push_back(src);
-  W.mark[src] = W.stamp;
-  W.dist[src] = 0;
-  size_t h = 0;
-  int last = src;
-  nlev = 1;
-  while (h < order.size()) {
-    int v = order[h++];
-    last = v;
-    for (int p = W.rowp[v]; p < W.rowp[v + 1]; p++) {
-      int w = W.cols[p];
-      if (W.part[w] != id || W.mark[w] == W.stamp) continue;
-      W.mark[w] = W.stamp;
-      W.dist[w] = W.dist[v] + 1;
-      nlev = std::max(nlev, W.dist[w] + 1);
-      order.push_back(w);
-    }
-  }
-  return last;
-}
-
-}  // namespace
-
-void nd_ordering(int n, const int* rowp, const int* cols, int* perm) {
-  NDWork W;
-  W.n = n;
-  W.rowp = rowp;
-  W.cols = cols;
-  W.mark.assign(n, 0);
-  W.dist.assign(n, 0);
-  W.part.assign(n, 0);
-  const int leaf = 200;
-  int next_id = 1;
-  vector<int> out;  // elimination order, built back to front per subproblem
-  out.reserve(n);
-
-  // Explicit stack of (vertex list, id); separators are emitted after both
-  // halves, so process as: push sep, then halves (halves ordered first).
-  struct Task {
-    vector<int> verts;
-    int id;
-  };
-  vector<int> all(n);
-  std::iota(all.begin(), all.end(), 0);
-  // result segments: we build a list of (vertices) in final order via
-  // recursion emulated with an explicit stack of actions.
-  struct Action {
-    bool emit;
-    vector<int> verts;
-    int id;
-  };
-  vector<Action> stack;
-  stack.push_back({false, all, 0});
-  vector<int> order, comp;
-  while (!stack.empty()) {
-    Action act = std::move(stack.back());
-    stack.pop_back();
-    if (act.emit) {
-      for (int v : act.verts) out.push_back(v);
-      continue;
-    }
-    vector<int>& V = act.verts;
-    const int id = act.id;
-    if ((int)V.size() <= leaf) {
-      // Order the leaf with AMD on its induced subgraph.
-      vector<int> loc(V.size());
-      vector<int> lp(V.size() + 1, 0), lc;
-      for (size_t i = 0; i < V.size(); i++) W.dist[V[i]] = (int)i;
-      for (size_t i = 0; i < V.size(); i++) {
-        int v = V[i];
-        for (int p = rowp[v]; p < rowp[v + 1]; p++)
-          if (W.part[cols[p]] == id) lc.push_back(W.dist[cols[p]]);
-        lp[i + 1] = (int)lc.size();
-      }
-      amd_ordering((int)V.size(), lp.data(), lc.data(), loc.data());
-      vector<int> ord(V.size());
-      for (size_t i = 0; i < V.size(); i++) ord[i] = V[loc[i]];
-      stack.push_back({true, ord, id});
-      continue;
-    }
-    // Connected components first: each is dissected on its own.
-    {
-      int nl;
-      nd_bfs(W, V[0], id, nl, order);
-      if (order.size() < V.size()) {
-        W.stamp++;
-        int st = W.stamp;
-        for (int v : order) W.mark[v] = st;
-        vector<int> rest;
-        for (int v : V)
-          if (W.mark[v] != st) rest.push_back(v);
-        int ida = next_id++, idb = next_id++;
-        for (int v : order) W.part[v] = ida;
-        for (int v : rest) W.part[v] = idb;
-        vector<int> a(order);
-        stack.push_back({false, rest, idb});
-        stack.push_back({false, a, ida});
-        continue;
-      }
-    }
-    // Pseudo-peripheral start: repeat BFS from the farthest vertex.
-    int s = V[0], nl = 0, prev_nl = -1;
-    for (int it = 0; it < 4; it++) {
-      int far = nd_bfs(W, s, id, nl, order);
-      if (nl <= prev_nl) break;
-      prev_nl = nl;
-      s = far;
-    }
-    nd_bfs(W, s, id, nl, order);
-    // Pick the level whose cumulative count crosses half, then scan a few
-    // neighbouring levels for the smallest separator with balance >= 1/4.
-    vector<int> cnt(nl, 0);
-    for (int v : order) cnt[W.dist[v]]++;
-    vector<long> cum(nl + 1, 0);
-    for (int l = 0; l < nl; l++) cum[l + 1] = cum[l] + cnt[l];
-    const long tot = (long)order.size();
-    int best = -1;
-    double best_score = 1e300;
-    for (int l = 1; l < nl - 1; l++) {
-      long below = cum[l], above = tot - cum[l + 1];
-      long small = std::min(below, above);
-      if (small * 4 < tot - cnt[l]) continue;
-      double imb = std::fabs((double)below - (double)above) / (double)tot;
-      double score = cnt[l] * (1.0 + 2.0 * imb);
-      if (score < best_score) {
-        best_score = score;
-        best = l;
-      }
-    }
-    if (best < 0) {
-      best = 1;
-      while (best < nl - 1 && cum[best + 1] * 2 < tot) best++;
-      if (best >= nl - 1) {
-        // Path-like or tiny graph: order directly.
-        vector<int> loc(V.size());
-        vector<int> lp(V.size() + 1, 0), lc;
-        for (size_t i = 0; i < V.size(); i++) W.dist[V[i]] = (int)i;
-        for (size_t i = 0; i < V.size(); i++) {
-          int v = V[i];
-          for (int p = rowp[v]; p < rowp[v + 1]; p++)
-            if (W.part[cols[p]] == id) lc.push_back(W.dist[cols[p]]);
-          lp[i + 1] = (int)lc.size();
-        }
-        amd_ordering((int)V.size(), lp.data(), lc.data(), loc.data());
-        vector<int> ord(V.size());
-        for (size_t i = 0; i < V.size(); i++) ord[i] = V[loc[i]];
-        stack.push_back({true, ord, id});
-        continue;
-      }
-    }
-    // Separator = level `best`, thinned: a separator vertex with no
-    // neighbour beyond the separator joins the near side.
-    vector<int> A, B, S;
-    int ida = next_id++, idb = next_id++, ids = next_id++;
-    for (int v : order) {
-      int d = W.dist[v];
-      if (d < best) A.push_back(v);
-      else if (d > best) B.push_back(v);
-    }
-    for (int v : order) {
-      if (W.dist[v] != best) continue;
-      bool touches_far = false;
-      for (int p = rowp[v]; p < rowp[v + 1]; p++) {
-        int w = cols[p];
-        if (W.part[w] == id && W.mark[w] == W.stamp && W.dist[w] == best + 1) {
-          touches_far = true;
-          break;
-        }
-      }
-      if (touches_far) S.push_back(v);
-      else A.push_back(v);
-    }
-    for (int v : A) W.part[v] = ida;
-    for (int v : B) W.part[v] = idb;
-    for (int v : S) W.part[v] = ids;
-    stack.push_back({true, S, ids});
-    stack.push_back({false, B, idb});
-    stack.push_back({false, A, ida});
-  }
-  for (int i = 0; i < n; i++) perm[i] = out[i];
-}
-
-// ===========================================================================
 // Elimination tree, post-order, column counts, supernodes.
 // ===========================================================================
 namespace {

@@ -30,10 +30,18 @@ struct SymbolicResult {
 // constraint, on the pattern with the diagonal added (ordering_utils.h:313-360).
 void amd_ordering(int n, const int* rowp, const int* cols, int* perm);
 
-// Nested dissection on the graph of the pattern (no METIS in this build):
-// multilevel-free recursive bisection with BFS level-set separators, leaves
-// ordered by AMD.  perm[new] = old.
+// Nested dissection on the graph of the pattern (no METIS in this build;
+// nd_ordering.cpp): multilevel node bisection against BFS level-set
+// separators, both refined by a band max-flow cut and node
+// Fiduccia-Mattheyses, leaves ordered by AMD.  perm[new] = old.
 void nd_ordering(int n, const int* rowp, const int* cols, int* perm);
+
+// From an analysed order S, a new order (perm[new] = old) with the columns of
+// every fundamental supernode rearranged so that columns predicted from the
+// values `vals` (threshold ustab) to form 2x2 pivots are adjacent.  Fill and
+// supernodes are those of S.
+void pair_partners(int n, const int* rowp, const int* cols, const double* vals, double ustab,
+                   const SymbolicResult& S, int* perm);
 
 // Full symbolic analysis.  `perm` may be null (natural order).
 void analyze(int n, const int* rowp, const int* cols, const int* perm,

@@ -92,6 +92,7 @@ class Options(ctypes.Structure):
         ("refine_tol", ctypes.c_double),
         ("max_refine", ctypes.c_int),
         ("perm", ctypes.POINTER(ctypes.c_int)),
+        ("vals_hint", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -315,6 +316,15 @@ class SparseLDL:
         opts.refine_tol = float(refine_tol)
         opts.max_refine = int(max_refine)
         self._perm = None
+        self._hint = None
+        if order == OrderingType.NESTED_DISSECTION and perm is None:
+            # the matrix values at construction order the columns inside
+            # each supernode (predicted 2x2 partners adjacent)
+            vals = mat.get_data()
+            if _is_torch(vals):
+                vals = vals.detach().cpu().numpy()
+            self._hint = np.ascontiguousarray(vals, dtype=np.float64)
+            opts.vals_hint = self._hint.ctypes.data_as(_f64p)
         if perm is not None:
             self._perm = np.ascontiguousarray(perm, dtype=np.int32)
             opts.perm = _ip(self._perm)

@@ -58,6 +58,13 @@ typedef struct {
   double refine_tol;    /* ||Kx-b||/||b|| target of kkt_ldl_solve_refined   */
   int max_refine;       /* refinement step limit                            */
   const int* perm;      /* KKT_LDL_ORDER_GIVEN: perm[new] = old             */
+  const double* vals_hint; /* optional values of the pattern (CSR order), read
+                            * only by kkt_ldl_create with KKT_LDL_ORDER_ND: the
+                            * columns of each separator are then ordered so
+                            * that predicted 2x2 pivot partners are adjacent
+                            * (the fill does not change).  The factorization's
+                            * pivot choices depend on it only through that
+                            * order.  NULL: separators in index order.         */
 } kkt_ldl_options;
 
 typedef struct {

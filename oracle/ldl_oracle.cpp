@@ -730,6 +730,11 @@ static int first_absmax(int n, const double* x, int inc) {  // idamax, 0-based
 }
 
 // Blocked threshold LDL^T of one front (sparse_ldl.h:1316-1553).
+// Pivot kinds chosen in fronts of more than 96 rows by the last factor:
+// 1x1 at the candidate, 2x2 with the next candidate, 1x1 at a later row,
+// 2x2 with a later row, delay (diagnostic counters for orderings).
+static long long g_pivot_kinds[5];
+
 static void factor_front(Solver& s, int ks, int fs, int m, int* vars,
                          double* F) {
   const int ldf = m, ldw = m, n = m;
@@ -795,6 +800,7 @@ static void factor_front(Solver& s, int ks, int fs, int m, int* vars,
         }
       }
       if (step == 0) {  // delay candidate k
+        if (m > 96) g_pivot_kinds[4]++;
         if (k < nc - 1) {
           sym_swap(vars, F, ldf, k, nc - 1);
           for (int c = 0; c < kw; c++) std::swap(W[k + c * ldw], W[nc - 1 + c * ldw]);
@@ -802,6 +808,7 @@ static void factor_front(Solver& s, int ks, int fs, int m, int* vars,
         nc -= 1;
         continue;
       }
+      if (m > 96) g_pivot_kinds[step == 1 ? (kpiv == k ? 0 : 2) : (kpiv == k + 1 ? 1 : 3)]++;
       int ksrc = k + step - 1;
       if (kpiv != ksrc) {
         sym_swap(vars, F, ldf, ksrc, kpiv);
@@ -1365,7 +1372,12 @@ void orc_get_etree(void* hh, int* parent, int* post, int* colcount) {
   std::copy(S.colcount.begin(), S.colcount.end(), colcount);
 }
 
+void orc_pivot_kinds(long long* out) {
+  for (int i = 0; i < 5; i++) out[i] = orc::g_pivot_kinds[i];
+}
+
 int orc_factor(void* hh, const double* vals, const double* diag) {
+  for (int i = 0; i < 5; i++) orc::g_pivot_kinds[i] = 0;
   OrcHandle* h = (OrcHandle*)hh;
   orc::factor(h->s, h->rowp.data(), h->cols.data(), vals, diag);
   return h->s.info;
