@@ -42,6 +42,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -1260,6 +1261,7 @@ __global__ void k_residual(int n, const int* rowp, const int* cols,
 // Sum of squares of each column: partial sums per (column, block), then a
 // fixed-order final sum per column (deterministic, no atomics).
 constexpr int kNormBlocks = 128;
+constexpr int kMaxNormRhs = 64;  // right-hand sides whose norms use fixed host buffers
 
 __global__ void k_colnorm2_part(int n, const double* x, int ld, double* part) {
   __shared__ double red[kTPB / kWave];
@@ -2048,6 +2050,12 @@ static void ensure_device(Handle& H) {
   H.uploaded = true;
 }
 
+static void ensure_u(Handle& H, int nrhs) {
+  if (H.u_nrhs >= nrhs) return;
+  H.d_U.alloc((size_t)H.u_total * nrhs);
+  H.u_nrhs = nrhs;
+}
+
 // vals == nullptr: refactorization with the values of the last call, which
 // stay on the device (only the diagonal is new).
 static int factor_impl(Handle& H, const double* vals, const double* diag, int mem,
@@ -2098,6 +2106,13 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
     upload_static(H);
     H.stats.replans++;
   }
+  // solve buffers for at least one right-hand side, so that a solve that
+  // may not allocate (kkt_ldl_solve_refined_noalloc) finds them in place
+  ensure_u(H, 1);
+  if (H.y_nrhs < 1) {
+    H.d_Y.alloc((size_t)H.i_total);
+    H.y_nrhs = 1;
+  }
   // zero and non-finite pivots of the (successful) factorization
   {
     DevPlan P = dev_plan(H);
@@ -2141,11 +2156,6 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
   return 0;
 }
 
-static void ensure_u(Handle& H, int nrhs) {
-  if (H.u_nrhs >= nrhs) return;
-  H.d_U.alloc((size_t)H.u_total * nrhs);
-  H.u_nrhs = nrhs;
-}
 
 template <class T>
 static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStream_t st) {
@@ -2238,14 +2248,22 @@ static void run_solve_levels(Handle& H, double* dx, int ldx, int nrhs, hipStream
   else run_solve_levels_t<double>(H, dx, ldx, nrhs, st);
 }
 
+// Buffers of the refinement: right-hand side copy, residual / correction,
+// previous iterate (each ldx * nrhs) and norm partials ((kNormBlocks + 2) *
+// nrhs).  Given by kkt_ldl_solve_refined_noalloc (its static scratch);
+// otherwise the handle's own buffers, allocated on demand.
+struct RefineWork {
+  double *b, *r, *xb, *norm;
+};
+
 static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
-                      hipStream_t st, bool refine) {
+                      hipStream_t st, bool refine, const RefineWork* ws = nullptr) {
   if (!H.factored) throw Error(-3, "solve called before factor");
   if (nrhs <= 0) return 0;
   if (ldx < H.n) throw Error(-1, "ldx < n");
   const size_t tot = (size_t)ldx * nrhs;
   double* dx = x;
-  if (mem == KKT_LDL_HOST || refine) {
+  if (!ws && (mem == KKT_LDL_HOST || refine)) {
     H.d_x.alloc(tot);
     HIPCHECK(hipMemcpyAsync(H.d_x.p, x, tot * sizeof(double),
                             mem == KKT_LDL_HOST ? hipMemcpyHostToDevice
@@ -2253,24 +2271,40 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
     dx = H.d_x.p;
   }
   int status = 0;
-  prof_reset(H);
+  if (!ws) prof_reset(H);  // (no event is destroyed by the scratch path)
   HIPCHECK(hipEventRecord(H.ev2, st));
   if (refine) {
-    H.d_b.alloc(tot);
-    H.d_r.alloc(tot);
-    H.d_xb.alloc(tot);
-    H.d_norm.alloc(2 * (size_t)nrhs + (size_t)kNormBlocks * nrhs);
-    HIPCHECK(hipMemcpyAsync(H.d_b.p, dx, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
-    std::vector<double> bn(nrhs), rn(nrhs);
-    double* part = H.d_norm.p + 2 * (size_t)nrhs;
+    RefineWork w;
+    if (ws) {
+      w = *ws;
+    } else {
+      H.d_b.alloc(tot);
+      H.d_r.alloc(tot);
+      H.d_xb.alloc(tot);
+      H.d_norm.alloc(2 * (size_t)nrhs + (size_t)kNormBlocks * nrhs);
+      w = RefineWork{H.d_b.p, H.d_r.p, H.d_xb.p, H.d_norm.p};
+    }
+    HIPCHECK(hipMemcpyAsync(w.b, dx, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // host copies of the squared norms (fixed size: no allocation here)
+    double bn[kMaxNormRhs], rn[kMaxNormRhs];
+    std::vector<double> bnv, rnv;
+    double* bnp = bn;
+    double* rnp = rn;
+    if (nrhs > kMaxNormRhs) {
+      bnv.resize(nrhs);
+      rnv.resize(nrhs);
+      bnp = bnv.data();
+      rnp = rnv.data();
+    }
+    double* part = w.norm + 2 * (size_t)nrhs;
     auto colnorm2 = [&](const double* v, double* out) {
       hipLaunchKernelGGL(k_colnorm2_part, dim3(kNormBlocks, nrhs), dim3(kTPB), 0, st, H.n, v,
                          ldx, part);
       hipLaunchKernelGGL(k_colnorm2_final, dim3((nrhs + 63) / 64), dim3(64), 0, st, nrhs,
                          (const double*)part, out);
     };
-    colnorm2(H.d_b.p, H.d_norm.p);
-    HIPCHECK(hipMemcpyAsync(bn.data(), H.d_norm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    colnorm2(w.b, w.norm);
+    HIPCHECK(hipMemcpyAsync(bnp, w.norm, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     run_solve_levels(H, dx, ldx, nrhs, st);
     int it = 0;
     double best = std::numeric_limits<double>::infinity();
@@ -2279,27 +2313,27 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
       {
         ProfScope ps_(H, st, KC_REFINE);
         hipLaunchKernelGGL(k_residual, rg, dim3(kTPB), 0, st, H.n, H.d_rowp.p, H.d_cols.p,
-                           H.d_vals.p, H.has_diag ? H.d_diag.p : nullptr, dx, H.d_b.p,
-                           H.d_r.p, ldx, nrhs);
+                           H.d_vals.p, H.has_diag ? H.d_diag.p : nullptr, dx, w.b,
+                           w.r, ldx, nrhs);
       }
-      colnorm2(H.d_r.p, H.d_norm.p);
-      HIPCHECK(hipMemcpyAsync(rn.data(), H.d_norm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+      colnorm2(w.r, w.norm);
+      HIPCHECK(hipMemcpyAsync(rnp, w.norm, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
-      const double w = refine_worst(bn.data(), rn.data(), nrhs);
-      const int act = refine_next(it, w, best, H.opt.refine_tol, H.opt.max_refine);
+      const double wr = refine_worst(bnp, rnp, nrhs);
+      const int act = refine_next(it, wr, best, H.opt.refine_tol, H.opt.max_refine);
       if (act == REFINE_UNDO_LAST) {
         // the last correction did not lower the residual: back to the
         // iterate before it, whose residual is `best`
-        HIPCHECK(hipMemcpyAsync(dx, H.d_xb.p, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIPCHECK(hipMemcpyAsync(dx, w.xb, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
         it--;
         break;
       }
-      best = w;
+      best = wr;
       if (act == REFINE_DONE) break;
-      run_solve_levels(H, H.d_r.p, ldx, nrhs, st);
+      run_solve_levels(H, w.r, ldx, nrhs, st);
       const size_t tb = ((size_t)H.n * nrhs + kTPB - 1) / kTPB;
-      hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)tb), dim3(kTPB), 0, st, H.n, dx, H.d_r.p,
-                         H.d_xb.p, ldx, nrhs);
+      hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)tb), dim3(kTPB), 0, st, H.n, dx, w.r,
+                         w.xb, ldx, nrhs);
     }
     H.stats.refine_steps = it;
     H.stats.residual = best;
@@ -2314,7 +2348,7 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
                                                 : hipMemcpyDeviceToDevice, st));
   HIPCHECK(hipStreamSynchronize(st));
   HIPCHECK(hipEventElapsedTime(&H.last_solve_ms, H.ev2, H.ev3));
-  prof_collect(H);
+  if (!ws) prof_collect(H);
   return status;
 }
 
@@ -2424,6 +2458,42 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
 using kktldl::Error;
 using kktldl::Handle;
 
+// --------------------------------------------------------------------------
+// Scratch of kkt_ldl_solve_refined_noalloc: the refinement's copy of the
+// right-hand sides, its residual / correction and previous iterate (each up
+// to kNoAllocEntries = ldx * nrhs entries) and the norm partials of up to
+// kNoAllocRhs columns: 25.2 MB, one copy per GPU, held until the process
+// ends.  In host code `scratch` is a stand-in for the symbol, not a device
+// address: the address is taken per device with hipGetSymbolAddress on the
+// entry point's first call there, and only that address is handed to
+// kernels and copies.  Every call on a GPU shares its copy, so calls that
+// overlap on one GPU race; each call writes every scratch entry it reads
+// (copy, residual, partials), so nothing has to start at zero.
+// --------------------------------------------------------------------------
+constexpr size_t kNoAllocEntries = size_t(1) << 20;
+constexpr int kNoAllocRhs = 64;
+constexpr size_t kScratchDoubles =
+    3 * kNoAllocEntries + (size_t)(kktldl::kNormBlocks + 2) * kNoAllocRhs;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[64] = {};  // device address of `scratch`, per device
+
+static double* scratch_address() {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw Error(-2, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(-2, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
 struct kkt_ldl_handle {
   Handle h;
 };
@@ -2508,6 +2578,42 @@ int kkt_ldl_solve_refined(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem,
   return guarded([&] {
     if (!h || !x) throw Error(-1, "null argument");
     return kktldl::solve_impl(h->h, nrhs, x, ldx, mem, (hipStream_t)stream, true);
+  });
+}
+
+int kkt_ldl_solve_refined_noalloc(kkt_ldl_t h, int nrhs, double* x, int ldx, void* stream) {
+  return guarded([&] {
+    if (!h || !x) throw Error(-1, "null argument");
+    Handle& H = h->h;
+    if (!H.factored) throw Error(-3, "solve called before factor");
+    if (nrhs <= 0) return 0;
+    if (ldx < H.n) throw Error(-1, "ldx < n");
+    if (nrhs > kNoAllocRhs || (size_t)ldx * (size_t)nrhs > kNoAllocEntries)
+      throw Error(-7, "kkt_ldl_solve_refined_noalloc: ldx * nrhs = " +
+                          std::to_string((size_t)ldx * (size_t)nrhs) + " with nrhs = " +
+                          std::to_string(nrhs) + " exceeds its scratch (" +
+                          std::to_string(kNoAllocEntries) + " entries, at most " +
+                          std::to_string(kNoAllocRhs) + " right-hand sides)");
+    if (H.u_nrhs < nrhs || H.y_nrhs < nrhs)
+      throw Error(-7, "kkt_ldl_solve_refined_noalloc: the handle's solve buffers hold " +
+                          std::to_string(std::min(H.u_nrhs, H.y_nrhs)) +
+                          " right-hand sides; one kkt_ldl_solve or kkt_ldl_solve_refined "
+                          "with at least " + std::to_string(nrhs) + " sizes them");
+    double* base = scratch_address();
+    const kktldl::RefineWork w{base, base + kNoAllocEntries, base + 2 * kNoAllocEntries,
+                               base + 3 * kNoAllocEntries};
+    // per-kernel timing creates events: off for this call
+    const bool prof = H.profiling;
+    H.profiling = false;
+    int st = 0;
+    try {
+      st = kktldl::solve_impl(H, nrhs, x, ldx, KKT_LDL_DEVICE, (hipStream_t)stream, true, &w);
+    } catch (...) {
+      H.profiling = prof;
+      throw;
+    }
+    H.profiling = prof;
+    return st;
   });
 }
 

@@ -129,7 +129,8 @@ class Stats(ctypes.Structure):
 EXPORTS = (
     "kkt_ldl_default_options", "kkt_ldl_create", "kkt_ldl_factor",
     "kkt_ldl_refactor", "kkt_ldl_set_pivot_threshold",
-    "kkt_ldl_solve", "kkt_ldl_solve_refined", "kkt_ldl_inertia",
+    "kkt_ldl_solve", "kkt_ldl_solve_refined", "kkt_ldl_solve_refined_noalloc",
+    "kkt_ldl_inertia",
     "kkt_ldl_get_perm", "kkt_ldl_get_sizes", "kkt_ldl_get_supernodes",
     "kkt_ldl_get_node_info", "kkt_ldl_get_stats", "kkt_ldl_last_times",
     "kkt_ldl_set_profiling", "kkt_ldl_get_kernel_times",
@@ -168,6 +169,7 @@ def load_library(path=LIB_PATH):
     lib.kkt_ldl_set_pivot_threshold.argtypes = [_vp, ctypes.c_double, ctypes.c_double]
     lib.kkt_ldl_solve.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp]
     lib.kkt_ldl_solve_refined.argtypes = lib.kkt_ldl_solve.argtypes
+    lib.kkt_ldl_solve_refined_noalloc.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp]
     lib.kkt_ldl_inertia.argtypes = [_vp, _i32p, _i32p, _i32p]
     lib.kkt_ldl_get_perm.argtypes = [_vp, _i32p, _i32p]
     lib.kkt_ldl_get_sizes.argtypes = [_vp, _i32p, _i32p]

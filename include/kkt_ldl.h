@@ -136,6 +136,21 @@ int kkt_ldl_solve(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem,
 int kkt_ldl_solve_refined(kkt_ldl_t h, int nrhs, double* x, int ldx, int mem,
                           void* stream);
 
+/* kkt_ldl_solve_refined for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): x is a DEVICE pointer (n x nrhs, leading dimension ldx), solved
+ * and refined in place, with the refinement's buffers in a static per-GPU
+ * scratch instead of handle buffers allocated on demand.  Limits:
+ * ldx * nrhs <= 1048576 and nrhs <= 64 (error -7 beyond); the handle's solve
+ * buffers must already hold nrhs right-hand sides (every kkt_ldl_factor
+ * sizes them for 1; a kkt_ldl_solve with a larger nrhs grows them), else
+ * error -7.  Calls no hipMalloc or hipFree; no per-kernel timing.  Calls that
+ * overlap on one GPU share the scratch and race.  No counterpart in the
+ * reference (which always allocates its solve temporaries,
+ * include/sparse_ldl.h:1829); it exists for the callers that cannot
+ * allocate.  Same results and status codes as kkt_ldl_solve_refined. */
+int kkt_ldl_solve_refined_noalloc(kkt_ldl_t h, int nrhs, double* x, int ldx, void* stream);
+
 /* Inertia from D of the last factorization; zero pivots count positive as in
  * the reference.  Replaces SparseLDL.get_inertia -- include/sparse_ldl.h:
  * 205-213, 1935-1987, amigo/amigo.cpp:728-731.  nzero (may be NULL) is the
