@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--mesh", type=int, default=256)
     ap.add_argument("--order", default="nd", choices=["nd", "amd"])
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32", "both"])
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run 5 device-to-device copies of MiB MiB (the known-size "
@@ -131,7 +131,7 @@ def run_gpu(P, precision, args, torch):
     kt_s = s.kernel_times()
     s.set_profiling(False)
     kt = {k: (kt_f[k][0] + kt_s[k][0], kt_f[k][1] + kt_s[k][1]) for k in kt_f}
-    return dict(s=s, ms=ms, status=status, inertia=inertia, residual=res, precision=precision,
+    return dict(s=s, ms=ms, status=status, inertia=inertia, residual=res, precision=precision, x=xh,
                 refine_steps=stats["refine_steps"], factor_ms=fac_ms,
                 solve_ms=sol_ms, t_analyze_s=t_analyze, stats=stats, kt=kt)
 
@@ -205,43 +205,73 @@ def bench_multi_rhs(r, P, torch, nrhs=64, reps=3):
     return out
 
 
+FACTOR_KERNELS = ("k_prepare", "k_assemble", "k_front_lds", "k_factor", "k_panel_rr", "k_panel",
+                  "k_update", "k_panel_inv", "k_panel_inv2", "k_pivot_check")
+SOLVE_KERNELS = ("k_forward", "k_backward", "k_fwd_step", "k_bwd_step", "k_bwd_gemv",
+                 "k_residual", "k_colnorm2_part", "k_colnorm2_final", "k_axpy_cols")
+
+
+def _kernel_base(name):
+    """Kernel name without namespace, template arguments or parameters."""
+    name = name.split("(")[0]
+    name = name.split("<")[0]
+    return name.split("::")[-1].strip()
+
+
 def roofline(r, P):
-    kt = r["kt"]
-    dom = max(kt, key=lambda k: kt[k][0])
-    t_ms, nl = kt[dom]
+    """The factorization against the matrix-core peak of its arithmetic
+    (FP32 MFMA for FP32 fronts, FP64 MFMA for FP64 fronts): achieved =
+    F_factor = sum_j c_j (c_j + 2) (fundamental structure) / the device time
+    of one factorization (HIP events on its stream).  Sub-objects: the MFMA
+    trailing update (algorithmic flops = the rank-1 Schur updates of the
+    fronts above 256 rows, over its kernel-only time), the refined solve
+    (algorithmic bytes per sweep: s nnz(L) + 4 sum(rows) + 16 n, plus 12
+    nnz(K) + 24 n per residual, against HBM) and the panel (diagnostic)."""
     s = r["s"]
-    esize = 4 if r["precision"] == "fp32" else 8
-    if dom == "update":
-        _, flops = large_front_work(s, P)
-        ach = flops / (t_ms * 1e-3) / 1e12
-        return dict(kernel="k_update", bound="mfma", achieved=ach, peak=FP64_PEAK_TFS,
-                    unit="TFLOP/s", frac=ach / FP64_PEAK_TFS, traffic=None,
-                    launches=nl, avg_launch_us=t_ms * 1e3 / max(nl, 1),
-                    algorithmic_per_launch=flops / max(nl, 1))
-    if dom in ("forward", "backward"):
-        byt = solve_bytes(s, esize) / 2.0 * (1 + r["refine_steps"])
-        ach = byt / (t_ms * 1e-3) / 1e9
-        return dict(kernel="k_" + dom, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=ach / HBM_PEAK_GBS, traffic=None, launches=nl,
-                    avg_launch_us=t_ms * 1e3 / max(nl, 1),
-                    algorithmic_per_launch=byt / max(nl, 1))
-    byt, _ = large_front_work(s, P, esize)
-    if dom != "panel":
-        byt = 2.0 * esize * s.stats()["nnz_l"]
-    ach = byt / (t_ms * 1e-3) / 1e9
-    # large-front panels: k_panel_rr (the panel's L in LDS) for every level
-    # whose fronts fit it -- all of them in the FP32 path of this workload --
-    # k_panel above that
-    name = {"panel": "k_panel_rr"}.get(dom, "k_" + dom)
-    return dict(kernel=name, bound="hbm",
-                achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                traffic=None, launches=nl, avg_launch_us=t_ms * 1e3 / max(nl, 1),
-                algorithmic_per_launch=byt / max(nl, 1))
+    st = r["stats"]
+    fp32 = r["precision"] == "fp32"
+    esize = 4 if fp32 else 8
+    peak = FP32_MFMA_PEAK_TFS if fp32 else FP64_PEAK_TFS
+    fac_s = r["factor_ms"] * 1e-3
+    ach = st["flops_factor"] / fac_s / 1e12
+    kt = r["kt"]
+    out = dict(kernel="factorization (every factor kernel, one factorization = one launch "
+                      "sequence on one stream)",
+               bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+               traffic=None, algorithmic_flops=st["flops_factor"],
+               avg_launch_ms=r["factor_ms"], launches=1)
+    byt_panel, upd_flops = large_front_work(s, P, esize)
+    t_upd, n_upd = kt["update"]
+    out["update"] = dict(kernel="k_update", bound="mfma", launches=n_upd,
+                         avg_launch_us=t_upd * 1e3 / max(n_upd, 1),
+                         algorithmic_flops=upd_flops,
+                         achieved=upd_flops / (t_upd * 1e-3) / 1e12, peak=peak, unit="TFLOP/s",
+                         frac=upd_flops / (t_upd * 1e-3) / 1e12 / peak)
+    t_pan, n_pan = kt["panel"]
+    out["panel"] = dict(kernel="k_panel_rr", launches=n_pan,
+                        avg_launch_us=t_pan * 1e3 / max(n_pan, 1),
+                        algorithmic_bytes_per_launch=byt_panel / max(n_pan, 1),
+                        note="16 B per remaining front entry per eliminated column; the "
+                             "panel is bound by its per-pivot dependency chain")
+    # refined solve: (1 + refine steps) sweeps pairs and as many residuals
+    sweeps = 2 * (1 + r["refine_steps"])
+    rows = int(np.sum(np.diff(s.get_supernodes()["node_ptr"]) +
+                      np.diff(s.get_supernodes()["contrib_ptr"])))
+    b_sweep = esize * st["nnz_l"] + 4.0 * rows + 16.0 * P.n
+    b_res = 12.0 * P.nnz + 24.0 * P.n
+    b_solve = sweeps * b_sweep + (1 + r["refine_steps"]) * b_res
+    sol_s = r["solve_ms"] * 1e-3
+    out["solve"] = dict(kernels="forward and backward sweeps, residual", bound="hbm",
+                        sweeps=sweeps, algorithmic_bytes=b_solve,
+                        achieved=b_solve / sol_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=b_solve / sol_s / 1e9 / HBM_PEAK_GBS, ms=r["solve_ms"])
+    return out
 
 
-def _pmc_pass(counter, child_args, timeout=180):
-    """Per-dispatch values of one PMC counter for this script run with
-    child_args under rocprofv3 (a child process), as {kernel name: [KiB]}."""
+def _pmc_pass(counters, child_args, timeout=180):
+    """Per-dispatch values of PMC counters for this script run with
+    child_args under rocprofv3 (a child process), as
+    {counter: {kernel base name: [value per dispatch]}}."""
     import csv
     import shutil
     import subprocess
@@ -251,38 +281,56 @@ def _pmc_pass(counter, child_args, timeout=180):
     if prof is None:
         raise RuntimeError("rocprofv3 not found")
     d = tempfile.mkdtemp(prefix="kkt_pmc_")
-    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
-           sys.executable, os.path.abspath(__file__)] + child_args
+    cmd = [prof, "--pmc"] + list(counters) + ["--output-format", "csv", "-d", d, "-o", "run",
+                                              "--", sys.executable,
+                                              os.path.abspath(__file__)] + child_args
     subprocess.run(cmd, check=True, timeout=timeout, stdout=subprocess.DEVNULL,
                    stderr=subprocess.DEVNULL, env=dict(os.environ, TMPDIR="/tmp"), cwd="/tmp")
     out = {}
     with open(os.path.join(d, "run_counter_collection.csv")) as f:
         for row in csv.DictReader(f):
-            out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+            out.setdefault(row["Counter_Name"], {}).setdefault(
+                _kernel_base(row["Kernel_Name"]), []).append(float(row["Counter_Value"]))
     shutil.rmtree(d, ignore_errors=True)
     return out
 
 
-def pmc_traffic(kernel, precision):
-    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE) in one
-    factor + refined solve of the headline precision, each counter scaled by
-    what it reads for device copies of known size (5 x 256 MiB)."""
+def pmc_measure(precision, order):
+    """Counters of one factor + refined solve of the headline precision
+    (the --pmc-probe program): HBM bytes (FETCH_SIZE + WRITE_SIZE, each
+    scaled by what it reads for device copies of known size, 5 x 256 MiB)
+    summed over the factor kernels and over the solve kernels, per launch of
+    k_panel_rr, and the MFMA work and utilisation of k_update (MfmaFlopsF32 /
+    MfmaFlopsF64 = 512 MOPS, MfmaUtil = MFMA-busy SIMD cycles over all SIMD
+    cycles of the dispatch, in %)."""
     copy_bytes = 256 << 20
+    probe = ["--pmc-probe", precision, "--order", order]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        cal = _pmc_pass(counter, ["--copy-calibration", "256"])
+        cal = _pmc_pass([counter], ["--copy-calibration", "256"])[counter]
         cv = [v for k, vs in cal.items() if "copyBuffer" in k for v in vs]
         if not cv:
             raise RuntimeError("no copy kernel in the calibration pass")
         scale = copy_bytes / 1024.0 / (sum(cv) / len(cv))
-        run = _pmc_pass(counter, ["--pmc-probe", precision, "--order", "nd"])
-        pat = re.compile(r"\b" + re.escape(kernel) + r"[<(]")
-        kv = [v for k, vs in run.items() if pat.search(k) for v in vs]
-        if not kv:
-            raise RuntimeError(f"no {kernel} dispatch in the PMC pass")
-        res[counter] = dict(kib_per_launch=sum(kv) / len(kv), scale=scale, launches=len(kv))
-    byt = sum(1024.0 * v["kib_per_launch"] * v["scale"] for v in res.values())
-    return byt, res
+        run = _pmc_pass([counter], probe)[counter]
+        res[counter] = dict(scale=scale, per_kernel={k: 1024.0 * scale * sum(v)
+                                                     for k, v in run.items()},
+                            launches={k: len(v) for k, v in run.items()})
+    mf = "MfmaFlopsF32" if precision == "fp32" else "MfmaFlopsF64"
+    mm = _pmc_pass([mf, "MfmaUtil"], probe)
+    upd_flops = sum(mm.get(mf, {}).get("k_update", []))
+    util = mm.get("MfmaUtil", {}).get("k_update", [])
+
+    def total(kernels):
+        return sum(res[c]["per_kernel"].get(k, 0.0) for c in res for k in kernels)
+
+    n_panel = res["FETCH_SIZE"]["launches"].get("k_panel_rr", 0)
+    return dict(factor_bytes=total(FACTOR_KERNELS), solve_bytes=total(SOLVE_KERNELS),
+                panel_bytes_per_launch=total(("k_panel_rr",)) / max(n_panel, 1),
+                update_counter_flops=upd_flops,
+                update_mfma_util_pct=float(np.mean(util)) if util else None,
+                scales={c: res[c]["scale"] for c in res},
+                counters=["FETCH_SIZE", "WRITE_SIZE", mf, "MfmaUtil"])
 
 
 def bench_assembly(args, torch, reps=50, cpu=True):
@@ -393,12 +441,36 @@ def bench_condense(args, torch, reps=50, cpu=True):
     return out
 
 
-def cpu_baseline(P, perm, reps):
-    """The reference's SparseLDL (oracle/_ref, compiled from its own headers)
-    on the same KKT with the same elimination order (the matrix permuted by
-    the engine's ordering, analysed with NATURAL), else the oracle."""
+def host_cpu():
+    """CPU model, sockets and cores of this host (lscpu), and the cores the
+    process may run on."""
+    info = {}
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return dict(model=info.get("Model name"), sockets=info.get("Socket(s)"),
+                cores_per_socket=info.get("Core(s) per socket"),
+                threads_per_core=info.get("Thread(s) per core"), usable_cpus=cores)
+
+
+def cpu_baseline(P, perm, reps, x_gpu=None):
+    """The reference's own SparseLDL (oracle/_ref, compiled from its headers
+    in the build container and shipped beside this script) on the same KKT
+    with the same elimination order (the matrix permuted by the engine's
+    ordering, analysed with NATURAL), with OpenMP on every core the process
+    may use (at most 16, the host share of one GPU here); one warm-up
+    factor + solve, then the median of reps.  Without the compiled reference,
+    the oracle (single-threaded restatement).  Also the relative difference
+    between the GPU solution and the reference's FP64 LDL^T solution."""
     import scipy.sparse as sp
-    threads = min(os.cpu_count() or 1, 16)
+    cpu = host_cpu()
+    threads = min(int(cpu["usable_cpus"] or 1), 16)
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     from oracle import pyoracle as po
 
@@ -412,19 +484,152 @@ def cpu_baseline(P, perm, reps):
         solver = po.RefLDL(P.n, Kp.indptr, Kp.indices, order=po.NATURAL)
     else:
         solver = po.OracleLDL(P.n, Kp.indptr, Kp.indices, order=po.NATURAL)
+        threads = 1
     solver.factor(Kp.data, dp)
-    solver.solve(bp)
+    xp = solver.solve(bp)
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
         solver.factor(Kp.data, dp)
-        solver.solve(bp)
+        xp = solver.solve(bp)
         times.append((time.perf_counter() - t0) * 1e3)
     inertia = solver.inertia()
-    return dict(value=float(np.median(times)), unit="ms", cores=threads, kind=kind,
-                sample=(f"{reps} factor+solve runs (median) after 1 warm-up of the same "
-                        f"n={P.n} KKT and elimination order, OMP threads={threads}"),
-                inertia=list(inertia))
+    out = dict(value=float(np.median(times)), unit="ms", cores=threads, kind=kind,
+               sample=(f"median of {reps} factor+solve runs after 1 warm-up, the same "
+                       f"n={P.n} KKT and elimination order, OMP threads={threads}"),
+               cpu=cpu, inertia=list(inertia))
+    if x_gpu is not None:
+        xr = np.empty(P.n)
+        xr[perm] = xp
+        out["solution"] = xr
+    return out
+
+
+def _solution_error(x, xref):
+    return float(np.linalg.norm(x - xref) / np.linalg.norm(xref))
+
+
+def bench_c1(args, torch, reps=5, cpu_reps=10):
+    """Config 1: the trapezoidal cart-pole KKT (N = 20,000 nodes, n =
+    179,996, AMD ordering, small fronts) with a plain FP64 factor + solve on
+    the GPU against the reference's SparseLDL on the host cores, same matrix
+    and elimination order.  Reports launches per factorization and levels:
+    this configuration is launch-bound."""
+    from amigo_amd import workloads as W
+    from amigo_amd.ldl import FP64, CSRMat, OrderingType, SparseLDL
+
+    P = W.cartpole()
+    vals = torch.tensor(P.vals, device="cuda")
+    diag = torch.tensor(P.diag, device="cuda")
+    rhs = torch.tensor(P.rhs, device="cuda")
+    x = torch.empty_like(rhs)
+    s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, vals), order=OrderingType.AMD,
+                  precision=FP64, refine_tol=1e-10)
+
+    def step():
+        s.factor(diag)
+        x.copy_(rhs)
+        s.solve(x, refine=True)
+        return s.get_inertia()
+
+    step()
+    torch.cuda.synchronize()
+    tt = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        inertia = step()
+        torch.cuda.synchronize()
+        tt.append((time.perf_counter() - t0) * 1e3)
+    xh = x.cpu().numpy()
+    res = float(np.linalg.norm(P.matrix() @ xh - P.rhs) / np.linalg.norm(P.rhs))
+    s.set_profiling(True)
+    s.factor(diag)
+    kf = s.kernel_times()
+    s.set_profiling(False)
+    st = s.stats()
+    out = dict(workload="cart-pole trapezoidal collocation, N=20000, n=179996, AMD, FP64",
+               gpu_ms=round(float(np.median(tt)), 3), residual=res, inertia=list(inertia),
+               refine_steps=st["refine_steps"], num_levels=st["num_levels"],
+               max_front=st["max_front"],
+               factor_launches=int(sum(v[1] for v in kf.values())))
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(P, s.get_perm()[0], cpu_reps, x_gpu=xh)
+        out["cpu_reference_ms"] = round(cb["value"], 3)
+        out["cpu_kind"], out["cpu_cores"] = cb["kind"], cb["cores"]
+        out["inertia_matches_reference"] = cb["inertia"] == list(inertia)
+        out["error_vs_reference"] = _solution_error(xh, cb["solution"])
+        out["speedup_vs_cpu"] = round(cb["value"] / out["gpu_ms"], 2)
+    return out
+
+
+def bench_c4(torch, reps=3):
+    """Config 4: post-optimality derivatives dJ/dp on the C2 KKT for p =
+    1,024 bilinear target coefficients and 64 seeded output functionals: the
+    64 adjoint solves in one refined call on the stored FP64 factors plus
+    dfdp = (dg/dp)^T psi, median of reps."""
+    from amigo_amd import workloads as W
+    from amigo_amd.adjoint import post_optimality_derivatives
+    from amigo_amd.ldl import FP64, CSRMat, OrderingType, SparseLDL
+
+    P, S = W.poisson2d_sensitivity(N=256, ncoarse=32, nout=64)
+    s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
+                  order=OrderingType.NESTED_DISSECTION, precision=FP64)
+    s.factor(P.diag)
+    post_optimality_derivatives(s, S["C"], S["dgdp"])
+    torch.cuda.synchronize()
+    tt = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        d = post_optimality_derivatives(s, S["C"], S["dgdp"])
+        torch.cuda.synchronize()
+        tt.append((time.perf_counter() - t0) * 1e3)
+    return dict(workload="C2 KKT, p=1024 (32x32 bilinear), 64 functionals, FP64 factors",
+                ms_64_adjoints=round(float(np.median(tt)), 3), shape=list(d.shape),
+                refine_steps=s.stats()["refine_steps"])
+
+
+def bench_ic(torch, reps=3):
+    """An inertia-correction iteration of 11 attempts (the reference's
+    maximum, inertia_correction.py:280-383): the first attempt factors
+    K + Sigma, each retry refactors with delta_w added on the primal
+    positions and the inertia read back, on C1 (FP64) and C2 (FP32 fronts).
+    Retries by kkt_ldl_refactor (values kept on the device) against full
+    kkt_ldl_factor calls; median ms of the 11 attempts over reps."""
+    from amigo_amd import workloads as W
+    from amigo_amd.ldl import FP32_REFINE, FP64, CSRMat, OrderingType, SparseLDL
+
+    out = {}
+    for name, gen, prec, order in (("c1", W.cartpole, FP64, OrderingType.AMD),
+                                   ("c2", W.poisson2d, FP32_REFINE,
+                                    OrderingType.NESTED_DISSECTION)):
+        P = gen()
+        vals = torch.tensor(P.vals, device="cuda")
+        s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, vals), order=order, precision=prec)
+        diags = []
+        dw = 1e-4
+        for a in range(11):
+            d = P.diag.copy()
+            if a:
+                d[P.primal_idx] += dw
+                dw *= 8.0
+            diags.append(torch.tensor(d, device="cuda"))
+        res = {}
+        for mode in ("refactor", "factor"):
+            tt = []
+            for _ in range(reps + 1):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                s.factor(diags[0])
+                s.get_inertia()
+                for d in diags[1:]:
+                    (s.refactor if mode == "refactor" else s.factor)(d)
+                    s.get_inertia()
+                torch.cuda.synchronize()
+                tt.append((time.perf_counter() - t0) * 1e3)
+            res[mode + "_ms"] = round(float(np.median(tt[1:])), 3)
+        res["attempts"] = 11
+        out[name] = res
+    return out
 
 
 def main():
@@ -475,17 +680,47 @@ def main():
     cond = bench_condense(args, torch, cpu=not args.no_cpu_baseline)
     mrhs = {p: bench_multi_rhs(v, P, torch) for p, v in results.items()}
     host = bench_host_buffers(P, head, args)
+    c1 = bench_c1(args, torch)
+    c4 = bench_c4(torch)
+    ic = bench_ic(torch)
     rl = roofline(r, P)
     if not args.no_traffic:
         try:
-            t, detail = pmc_traffic(rl["kernel"], head)
-            rl["traffic"] = t
-            rl["traffic_over_algorithmic"] = t / rl["algorithmic_per_launch"]
-            rl["traffic_pmc"] = detail
+            m = pmc_measure(head, args.order)
+            rl["traffic"] = m["factor_bytes"]
+            rl["traffic_note"] = ("HBM bytes of one factorization: FETCH_SIZE + WRITE_SIZE over "
+                                  "every factor kernel, each scaled by a device copy of known "
+                                  "size")
+            rl["update"]["counter_flops"] = m["update_counter_flops"]
+            t_upd = rl["update"]["avg_launch_us"] * rl["update"]["launches"] * 1e-6
+            if m["update_counter_flops"]:
+                rl["update"]["counter_achieved"] = m["update_counter_flops"] / t_upd / 1e12
+            rl["update"]["mfma_util_pct"] = m["update_mfma_util_pct"]
+            rl["solve"]["traffic"] = m["solve_bytes"]
+            rl["solve"]["counter_GBs"] = m["solve_bytes"] / (r["solve_ms"] * 1e-3) / 1e9
+            rl["panel"]["traffic_per_launch"] = m["panel_bytes_per_launch"]
+            rl["panel"]["traffic_over_algorithmic"] = (
+                m["panel_bytes_per_launch"] / rl["panel"]["algorithmic_bytes_per_launch"])
+            rl["pmc"] = dict(scales=m["scales"], counters=m["counters"])
         except Exception as e:  # the counters are a diagnostic, not the measurement
-            print(f"PMC traffic unavailable: {e}", file=sys.stderr)
+            print(f"PMC counters unavailable: {e}", file=sys.stderr)
     perm = r["s"].get_perm()[0]
-    cpu = None if args.no_cpu_baseline else cpu_baseline(P, perm, args.cpu_reps)
+    xgpu = {p: v["x"] for p, v in results.items()}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(P, perm, args.cpu_reps, x_gpu=True)
+    prec_out = {}
+    for p, v in results.items():
+        d = {"ms_per_step": round(v["ms"], 3), "factor_ms": round(v["factor_ms"], 3),
+             "solve_refined_ms": round(v["solve_ms"], 3),
+             "refine_steps": v["refine_steps"], "residual": v["residual"],
+             "inertia": list(v["inertia"]), "status": v["status"],
+             "analyze_s": round(v["t_analyze_s"], 3)}
+        if "fp64" in xgpu and p != "fp64":
+            d["error_vs_fp64_gpu"] = _solution_error(xgpu[p], xgpu["fp64"])
+        if cpu and "solution" in cpu:
+            d["error_vs_reference_ldl"] = _solution_error(xgpu[p], cpu["solution"])
+        prec_out[p] = d
+    if cpu:
+        cpu.pop("solution", None)
     out = {
         "metric": METRIC,
         "value": round(r["ms"], 3),
@@ -506,21 +741,19 @@ def main():
             "num_nodes": r["stats"]["num_nodes"], "num_levels": r["stats"]["num_levels"],
             "max_front": r["stats"]["max_front"],
             "flops_factor": r["stats"]["flops_factor"], "nnz_l": r["stats"]["nnz_l"],
+            "crit_path_pivots": r["stats"]["crit_path_pivots"],
+            "level_panel_sum": r["stats"]["level_panel_sum"],
         },
-        "precisions": {
-            p: {"ms_per_step": round(v["ms"], 3), "factor_ms": round(v["factor_ms"], 3),
-                "solve_refined_ms": round(v["solve_ms"], 3),
-                "refine_steps": v["refine_steps"], "residual": v["residual"],
-                "inertia": list(v["inertia"]), "status": v["status"],
-                "analyze_s": round(v["t_analyze_s"], 3)}
-            for p, v in results.items()
-        },
+        "precisions": prec_out,
         "kernels": {k: {"ms": round(t, 3), "launches": n} for k, (t, n) in r["kt"].items()},
         "roofline": rl,
         "assembly": asm,
         "condensation": cond,
         "multi_rhs_64": mrhs,
         "host_buffers": host,
+        "c1_cartpole": c1,
+        "c4_adjoint": c4,
+        "inertia_correction_11": ic,
         "cpu_baseline": cpu,
     }
     if cpu and cpu.get("inertia"):

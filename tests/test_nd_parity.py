@@ -42,7 +42,7 @@ def _forest():
 def _case(name):
     if name == "forest":
         return _forest()
-    P = W.poisson2d()
+    P = W.elasticity3d(N=12) if name == "elasticity3d_12" else W.poisson2d()
     return P.n, P.rowp, P.cols, P.vals, P.diag, P.rhs
 
 
@@ -61,13 +61,18 @@ def _oracle(name, perm, n, rowp, cols, vals, diag, rhs):
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("name", ["poisson2d_256", "forest"])
+@pytest.mark.parametrize("name", ["poisson2d_256", "forest", "elasticity3d_12"])
 def test_nd_factor_matches_oracle_node_by_node(name, prec):
+    """elasticity3d_12 (n = 19,773) has fronts above 2,048 rows: in FP32 their
+    levels take the global-memory panel (k_panel), in FP64 every level above
+    1,024 rows does."""
     print("case", name, "precision", prec, flush=True)
     n, rowp, cols, vals, diag, rhs = _case(name)
     s = SparseLDL(CSRMat(n, n, rowp, cols, vals), order=OrderingType.NESTED_DISSECTION,
                   precision=prec, max_refine=30)
     assert s.factor(diag) == 0
+    if name == "elasticity3d_12":
+        assert s.stats()["max_front"] > 2048
     perm = s.get_perm()[0]
     inertia, ni, xo = _oracle(name, perm, n, rowp, cols, vals, diag, rhs)
     assert s.get_inertia() == inertia
