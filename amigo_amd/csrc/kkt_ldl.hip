@@ -2064,6 +2064,11 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
   if (mem != KKT_LDL_HOST && mem != KKT_LDL_DEVICE) throw Error(-1, "unknown memory space");
   if (!vals && !H.vals_loaded) throw Error(-3, "refactor called before factor");
   ensure_device(H);
+  // the large-front panel (vars/pre) and the solve's front gather and
+  // L21^T products stage a front's rows in LDS (two int or one double per row)
+  if ((size_t)((H.max_cap + 31) & ~31) * sizeof(double) > 160 * 1024)
+    throw Error(-6, "front of " + std::to_string(H.max_cap) +
+                        " rows exceeds the 20448 rows the LDS-staged panel and solve kernels hold");
   const hipMemcpyKind kind = mem == KKT_LDL_DEVICE ? hipMemcpyDeviceToDevice
                                                    : hipMemcpyHostToDevice;
   if (vals) {

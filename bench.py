@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--pmc-probe", choices=["fp64", "fp32"], default=None,
                     help="only run one factor + refined solve in this precision (the "
                          "program the PMC traffic passes profile) and exit")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
+                    help="c3: one FP32 factor + refined solve of the 40^3 elasticity KKT "
+                         "(n = 620,289) with its inertia and residual checks, then exit")
+    ap.add_argument("--c3-mesh", type=int, default=40)
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 --pmc passes behind roofline.traffic")
     return ap.parse_args()
@@ -632,6 +636,61 @@ def bench_ic(torch, reps=3):
     return out
 
 
+def run_c3(args, torch):
+    """Config 3: 3D linear-elasticity distributed control on a 40^3 hex mesh
+    (n = 620,289), nested dissection, FP32 fronts + FP64 refinement.  One
+    factorization and one refined solve (then a second, timed alike), the
+    inertia against the structural one (every row positive except the free
+    multipliers) and the relative residual; the front arena size."""
+    from amigo_amd import workloads as W
+    from amigo_amd.ldl import FP32_REFINE, CSRMat, OrderingType, SparseLDL
+
+    t0 = time.perf_counter()
+    P = W.elasticity3d(N=args.c3_mesh)
+    t_gen = time.perf_counter() - t0
+    print(f"c3: generated n={P.n} nnz={P.nnz} in {t_gen:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals), order=OrderingType.NESTED_DISSECTION,
+                  precision=FP32_REFINE, refine_tol=1e-10, max_refine=20)
+    t_an = time.perf_counter() - t0
+    st = s.stats()
+    print(f"c3: analysed in {t_an:.1f} s, max front {st['max_front']}, flops "
+          f"{st['flops_factor']:.3e}", file=sys.stderr, flush=True)
+    diag = torch.tensor(P.diag, device="cuda")
+    rhs = torch.tensor(P.rhs, device="cuda")
+    runs = []
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        status = s.factor(diag)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x = rhs.clone()
+        sst = s.solve(x, refine=True)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        runs.append(((t1 - t0) * 1e3, (t2 - t1) * 1e3, status, sst))
+        print(f"c3: factor {runs[-1][0]:.1f} ms solve {runs[-1][1]:.1f} ms", file=sys.stderr,
+              flush=True)
+    st = s.stats()
+    xh = x.cpu().numpy()
+    res = float(np.linalg.norm(P.matrix() @ xh - P.rhs) / np.linalg.norm(P.rhs))
+    nd = len(P.dual_idx)
+    out = dict(workload=f"3D elasticity distributed-control QP, {args.c3_mesh}^3 hex mesh, KKT "
+                        f"n={P.n}, nested dissection, FP32 fronts + FP64 refinement",
+               n=P.n, nnz=P.nnz, analyze_s=round(t_an, 2), max_front=st["max_front"],
+               flops_factor=st["flops_factor"], nnz_l=st["nnz_l"],
+               front_arena_GB=round(st["front_bytes"] / 1e9, 2),
+               factor_ms=round(runs[-1][0], 2), solve_refined_ms=round(runs[-1][1], 2),
+               ms_per_step=round(runs[-1][0] + runs[-1][1], 2),
+               refine_steps=st["refine_steps"], residual=res, factor_status=runs[-1][2],
+               solve_status=runs[-1][3], inertia=list(s.get_inertia()),
+               expected_inertia=[P.n - nd, nd], replans=st["replans"],
+               num_delayed=st["num_delayed"])
+    out["inertia_matches_structure"] = out["inertia"] == out["expected_inertia"]
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -650,6 +709,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": a.numel() * 8, "copies": 5}))
+        return
+    if args.workload == "c3":
+        run_c3(args, torch)
         return
     if args.pmc_probe:
         from amigo_amd.ldl import FP32_REFINE, FP64, CSRMat, OrderingType, SparseLDL
