@@ -52,6 +52,9 @@ def parse():
                     help="c3: one FP32 factor + refined solve of the 40^3 elasticity KKT "
                          "(n = 620,289) with its inertia and residual checks, then exit")
     ap.add_argument("--c3-mesh", type=int, default=40)
+    ap.add_argument("--probe-steps", type=int, default=1,
+                    help="steps (factor + refined solve) of the --pmc-probe program, after "
+                         "one untimed step when more than one")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 --pmc passes behind roofline.traffic")
     return ap.parse_args()
@@ -719,9 +722,13 @@ def main():
         order = OrderingType.NESTED_DISSECTION if args.order == "nd" else OrderingType.AMD
         s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, torch.tensor(P.vals, device="cuda")),
                       order=order, precision=FP32_REFINE if args.pmc_probe == "fp32" else FP64)
-        s.factor(torch.tensor(P.diag, device="cuda"))
-        x = torch.tensor(P.rhs, device="cuda")
-        s.solve(x, refine=True)
+        d = torch.tensor(P.diag, device="cuda")
+        b = torch.tensor(P.rhs, device="cuda")
+        for _ in range(args.probe_steps + (1 if args.probe_steps > 1 else 0)):
+            s.factor(d)
+            x = b.clone()
+            s.solve(x, refine=True)
+            s.get_inertia()
         torch.cuda.synchronize()
         return
     P = W.poisson2d(N=args.mesh)
