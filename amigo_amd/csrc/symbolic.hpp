@@ -38,8 +38,9 @@ void nd_ordering(int n, const int* rowp, const int* cols, int* perm);
 
 // From an analysed order S, a new order (perm[new] = old) with the columns of
 // every fundamental supernode rearranged so that columns predicted from the
-// values `vals` (threshold ustab) to form 2x2 pivots are adjacent.  Fill and
-// supernodes are those of S.
+// values `vals` (threshold ustab) to form 2x2 pivots are adjacent.  Each
+// supernode keeps its dense diagonal block and the structure below it; the
+// new order is analysed again by the caller.
 void pair_partners(int n, const int* rowp, const int* cols, const double* vals, double ustab,
                    const SymbolicResult& S, int* perm);
 
