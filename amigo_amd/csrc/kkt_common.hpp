@@ -213,20 +213,83 @@ struct RedBuf {
   double v2[NT / kWave];
 };
 
+// Cross-lane moves by DPP (VALU speed): quad_perm [1,0,3,2] and [2,3,0,1],
+// row_half_mirror and row_mirror pair every lane of a 16-lane row with every
+// other after four steps.
+template <int CTRL>
+__device__ inline int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = dpp_i<CTRL>((int)b);
+  const int hi = dpp_i<CTRL>((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double lane_d(double x, int l) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// (max |v| with the first row on ties, max of a second value); the order of
+// combination does not change the result.
+__device__ inline void am_combine(double& b, int& i, double& r, double ob, int oi, double orr) {
+  if (ob > b || (ob == b && oi < i)) {
+    b = ob;
+    i = oi;
+  }
+  r = fmax(r, orr);
+}
+template <int CTRL>
+__device__ inline void am_step(double& b, int& i, double& r) {
+  am_combine(b, i, r, dpp_d<CTRL>(b), dpp_i<CTRL>(i), dpp_d<CTRL>(r));
+}
+__device__ inline void row_argmax2(double& b, int& i, double& r) {
+  am_step<0xB1>(b, i, r);
+  am_step<0x4E>(b, i, r);
+  am_step<0x141>(b, i, r);
+  am_step<0x140>(b, i, r);
+}
+// wave-wide result, uniform in every lane
+__device__ inline void wave_argmax2(double& b, int& i, double& r) {
+  row_argmax2(b, i, r);
+  double rb = lane_d(b, 0), rr = lane_d(r, 0);
+  int ri = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+  for (int q = 1; q < kWave / 16; q++)
+    am_combine(rb, ri, rr, lane_d(b, 16 * q), __builtin_amdgcn_readlane(i, 16 * q),
+               lane_d(r, 16 * q));
+  b = rb;
+  i = ri;
+  r = rr;
+}
+__device__ inline double row_max(double x) {
+  x = fmax(x, dpp_d<0xB1>(x));
+  x = fmax(x, dpp_d<0x4E>(x));
+  x = fmax(x, dpp_d<0x141>(x));
+  x = fmax(x, dpp_d<0x140>(x));
+  return x;
+}
+__device__ inline double wave_max(double x) {
+  x = row_max(x);
+  double r = lane_d(x, 0);
+#pragma unroll
+  for (int q = 1; q < kWave / 16; q++) r = fmax(r, lane_d(x, 16 * q));
+  return r;
+}
+
 // Block-wide (max |x|, first index) and second plain max; results broadcast.
+// Each wave reduces by DPP, then the waves' results are combined in wave
+// order; the combination is order independent, so the result is the
+// lexicographic optimum whatever the wave count.
 template <int NT>
 __device__ inline void block_argmax2(double& val, int& idx, double& val2,
                                      RedBuf<NT>& rb) {
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    double ov = __shfl_xor(val, off);
-    int oi = __shfl_xor(idx, off);
-    double o2 = __shfl_xor(val2, off);
-    if (ov > val || (ov == val && oi < idx)) {
-      val = ov;
-      idx = oi;
-    }
-    val2 = fmax(val2, o2);
-  }
+  wave_argmax2(val, idx, val2);
+  if constexpr (NT == kWave) return;
   const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
   __syncthreads();
   if (l == 0) {
@@ -238,13 +301,7 @@ __device__ inline void block_argmax2(double& val, int& idx, double& val2,
   val = rb.v[0];
   idx = rb.i[0];
   val2 = rb.v2[0];
-  for (int t = 1; t < NT / kWave; t++) {
-    if (rb.v[t] > val || (rb.v[t] == val && rb.i[t] < idx)) {
-      val = rb.v[t];
-      idx = rb.i[t];
-    }
-    val2 = fmax(val2, rb.v2[t]);
-  }
+  for (int t = 1; t < NT / kWave; t++) am_combine(val, idx, val2, rb.v[t], rb.i[t], rb.v2[t]);
 }
 
 // --------------------------------------------------------------------------
@@ -622,73 +679,6 @@ __device__ inline void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Cross-lane moves by DPP (VALU speed): quad_perm [1,0,3,2] and [2,3,0,1],
-// row_half_mirror and row_mirror pair every lane of a 16-lane row with every
-// other after four steps.
-template <int CTRL>
-__device__ inline int dpp_i(int x) {
-  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
-}
-template <int CTRL>
-__device__ inline double dpp_d(double x) {
-  const long long b = __builtin_bit_cast(long long, x);
-  const int lo = dpp_i<CTRL>((int)b);
-  const int hi = dpp_i<CTRL>((int)(b >> 32));
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ inline double lane_d(double x, int l) {
-  const long long b = __builtin_bit_cast(long long, x);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// (max |v| with the first row on ties, max of a second value); the order of
-// combination does not change the result.
-__device__ inline void am_combine(double& b, int& i, double& r, double ob, int oi, double orr) {
-  if (ob > b || (ob == b && oi < i)) {
-    b = ob;
-    i = oi;
-  }
-  r = fmax(r, orr);
-}
-template <int CTRL>
-__device__ inline void am_step(double& b, int& i, double& r) {
-  am_combine(b, i, r, dpp_d<CTRL>(b), dpp_i<CTRL>(i), dpp_d<CTRL>(r));
-}
-__device__ inline void row_argmax2(double& b, int& i, double& r) {
-  am_step<0xB1>(b, i, r);
-  am_step<0x4E>(b, i, r);
-  am_step<0x141>(b, i, r);
-  am_step<0x140>(b, i, r);
-}
-// wave-wide result, uniform in every lane
-__device__ inline void wave_argmax2(double& b, int& i, double& r) {
-  row_argmax2(b, i, r);
-  double rb = lane_d(b, 0), rr = lane_d(r, 0);
-  int ri = __builtin_amdgcn_readlane(i, 0);
-#pragma unroll
-  for (int q = 1; q < kWave / 16; q++)
-    am_combine(rb, ri, rr, lane_d(b, 16 * q), __builtin_amdgcn_readlane(i, 16 * q),
-               lane_d(r, 16 * q));
-  b = rb;
-  i = ri;
-  r = rr;
-}
-__device__ inline double row_max(double x) {
-  x = fmax(x, dpp_d<0xB1>(x));
-  x = fmax(x, dpp_d<0x4E>(x));
-  x = fmax(x, dpp_d<0x141>(x));
-  x = fmax(x, dpp_d<0x140>(x));
-  return x;
-}
-__device__ inline double wave_max(double x) {
-  x = row_max(x);
-  double r = lane_d(x, 0);
-#pragma unroll
-  for (int q = 1; q < kWave / 16; q++) r = fmax(r, lane_d(x, 16 * q));
-  return r;
-}
 
 // One panel launch over the large fronts of a level (kkt_panel.hip): the
 // LDS-resident panel k_panel_rr where the panel's L fits LDS, else the

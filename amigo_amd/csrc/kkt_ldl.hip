@@ -393,10 +393,11 @@ __global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
 // v_mfma_f64_16x16x4f64, whose output is D[l/16 + 4r][l%16].
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
-                                                 const int* wtile, int base) {
+                                                 const int* wtile, const int* wbase) {
   typedef T vec4 __attribute__((ext_vector_type(4)));
   const int p = wnode[blockIdx.x];
   const int t = wtile[blockIdx.x];
+  const int base = wbase[blockIdx.x];
   // everything per node depends on p alone: one round of loads
   const int stat = P.stat[p];
   const PanelState st = P.ps[p];
@@ -1369,6 +1370,12 @@ struct Handle {
   // environment variable KKT_LDL_PANEL_SPECULATION=0 selects the sequential
   // panel everywhere (same factors, bit for bit)
   int panel_spec = 1;
+  // factor schedule: by default a node starts at the tick after its last
+  // child's last panel; the environment variable KKT_LDL_LEVEL_SCHEDULE=1
+  // selects the level schedule (every node of a tree level starts together
+  // and the level lasts as long as its longest panel chain).  Same factors,
+  // bit for bit.
+  bool level_sched = false;
   // optional per-kernel-class device timing (events around every launch)
   bool profiling = false;
   std::vector<hipEvent_t> prof_ev;   // pairs
@@ -1383,25 +1390,36 @@ struct Handle {
   std::vector<int> ns, cs, cap, mode, level, lvl_ptr, lvl_nodes;
   std::vector<long long> foff, ioff, woff, uoff, invoff;
   std::vector<int> invnom, am_ptr, am_src, am_row, am_col;
-  std::vector<int> asm_node, asm_chunk, asm_lvl_ptr;
+  std::vector<int> asm_node, asm_chunk;
   long long f_total = 0, i_total = 0, w_total = 0, u_total = 0;
   int max_cap = 0;
   // device
   DBuf<int> d_ns, d_cs, d_cap, d_vptr, d_cptr, d_chptr, d_ch, d_mode, d_n2v,
-      d_crows, d_invnom, d_amptr, d_amsrc, d_amrow, d_amcol, d_lvl_nodes, d_asm_node,
+      d_crows, d_invnom, d_amptr, d_amsrc, d_amrow, d_amcol, d_asm_node,
       d_asm_chunk, d_rowp, d_cols;
   DBuf<long long> d_invoff, d_foff, d_ioff, d_woff, d_uoff;
   DBuf<int> d_din, d_m, d_npiv, d_ndel, d_doff, d_stat, d_need, d_cnt;
   DBuf<PanelState> d_ps;
-  // per level: small fronts (one workgroup each) and large fronts with
-  // their per-panel update work lists
+  // per level (the solve's schedule): small fronts (one workgroup each),
+  // large fronts, LDS fronts, and the largest panel count of a level
   std::vector<int> small_ptr, small_nodes, large_ptr, large_nodes, large_panels;
-  std::vector<int> lds_ptr, lds_nodes, lds_cap;  // per level: LDS fronts, max cap
+  std::vector<int> lds_ptr, lds_nodes;
   int lds_front = kLdsFront;  // largest front dimension kept in LDS
-  DBuf<int> d_lds_nodes;
-  std::vector<int> upd_ptr, upd_node, upd_tile;  // upd_ptr per (level, panel)
-  std::vector<int> upd_lvl;                      // first upd_ptr slot of a level
-  DBuf<int> d_small_nodes, d_large_nodes, d_upd_node, d_upd_tile;
+  DBuf<int> d_large_nodes;
+  // The factorization's schedule, in ticks.  Tick t runs k_prepare, the LDS
+  // fronts, the assembly and the small fronts of the nodes that start at t
+  // (tk_grp: all of them; tk_lds, tk_small, tk_asm), then one panel launch
+  // over the large fronts active at t (tk_act, with the launch's largest
+  // slot and whether it holds a root front) and one update launch over
+  // their trailing tiles (upd_ptr per tick; upd_base: the first column of
+  // the tile grid, from the front's panel index at t).
+  int ntick = 0;
+  std::vector<int> tk_start;  // per node: start tick
+  std::vector<int> tk_grp_ptr, tk_grp_nodes, tk_lds_ptr, tk_lds_nodes, tk_small_ptr,
+      tk_small_nodes, tk_asm_ptr, tk_act_ptr, tk_act_nodes, tk_act_cap, tk_act_root;
+  std::vector<int> upd_ptr, upd_node, upd_tile, upd_base;
+  DBuf<int> d_tk_grp_nodes, d_tk_lds_nodes, d_tk_small_nodes, d_tk_act_nodes, d_upd_node,
+      d_upd_tile, d_upd_base;
   // solve: per level, the nodes solved by one workgroup each (sv), the
   // backward L21^T GEMV tiles of the large nodes (bg, 16 columns each) and
   // their panel steps (fs forward, bs backward: fs_lvl[l] is the first step
@@ -1492,61 +1510,30 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   H.f_total = f;
   H.i_total = iv;
   H.u_total = u;
-  // panel scratch: per level, nodes in the level share the arena
-  long long wmax = 0;
-  for (int l = 0; l < nlev; l++) {
-    long long w = 0;
-    for (int q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; q++) {
-      int k = H.lvl_nodes[q];
-      H.woff[k] = w;
-      w += (long long)H.cap[k] * kNB;
-    }
-    wmax = std::max(wmax, w);
-  }
-  H.w_total = wmax;
-  // assembly work items
-  H.asm_node.clear();
-  H.asm_chunk.clear();
-  H.asm_lvl_ptr.assign(nlev + 1, 0);
-  for (int l = 0; l < nlev; l++) {
-    for (int q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; q++) {
-      int k = H.lvl_nodes[q];
-      if (H.cap[k] <= H.lds_front) continue;  // assembled inside k_front_lds
-      int nch = (H.cap[k] + kAsmCols - 1) / kAsmCols;
-      for (int c = 0; c < nch; c++) {
-        H.asm_node.push_back(k);
-        H.asm_chunk.push_back(c);
-      }
-    }
-    H.asm_lvl_ptr[l + 1] = (int)H.asm_node.size();
-  }
+  // Per level (the solve's schedule): LDS fronts sorted by cap, small
+  // (Cholesky) fronts, large fronts and the level's largest panel count.
   // Large fronts get the multi-workgroup panel/update path.  A panel ends
   // when kw >= kNBLarge-1 or the candidates run out, so every panel but a
   // front's last takes at least kNBLarge-1 pivots: ceil(pivots / 15) panel
-  // launches suffice, and the update tiles of panel t may start at the
-  // smaller bound t*(kNBLarge-2).
+  // launches suffice, and the update tiles of a front's panel t may start
+  // at the smaller bound t*(kNBLarge-2).
+  auto is_large = [&](int k) { return H.cap[k] > H.lds_front && H.mode[k] != 2; };
+  auto panels_of = [&](int k) { return (H.cap[k] - H.cs[k] + kNBLarge - 2) / (kNBLarge - 1); };
   H.small_ptr.assign(nlev + 1, 0);
   H.large_ptr.assign(nlev + 1, 0);
   H.lds_ptr.assign(nlev + 1, 0);
-  H.lds_cap.assign(nlev, 0);
   H.small_nodes.clear();
   H.large_nodes.clear();
   H.lds_nodes.clear();
   H.large_panels.assign(nlev, 0);
-  H.upd_ptr.assign(1, 0);
-  H.upd_lvl.assign(nlev + 1, 0);
-  H.upd_node.clear();
-  H.upd_tile.clear();
   for (int l = 0; l < nlev; l++) {
     for (int q = H.lvl_ptr[l]; q < H.lvl_ptr[l + 1]; q++) {
       int k = H.lvl_nodes[q];
       if (H.cap[k] <= H.lds_front) {
         H.lds_nodes.push_back(k);
-        H.lds_cap[l] = std::max(H.lds_cap[l], H.cap[k]);
-      } else if (H.mode[k] != 2) {
+      } else if (is_large(k)) {
         H.large_nodes.push_back(k);
-        H.large_panels[l] =
-            std::max(H.large_panels[l], (H.cap[k] - H.cs[k] + kNBLarge - 2) / (kNBLarge - 1));
+        H.large_panels[l] = std::max(H.large_panels[l], panels_of(k));
       } else {
         H.small_nodes.push_back(k);
       }
@@ -1556,21 +1543,132 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     H.lds_ptr[l + 1] = (int)H.lds_nodes.size();
     H.small_ptr[l + 1] = (int)H.small_nodes.size();
     H.large_ptr[l + 1] = (int)H.large_nodes.size();
-    H.upd_lvl[l] = (int)H.upd_ptr.size() - 1;
-    for (int t = 0; t < H.large_panels[l]; t++) {
-      const int base = t * (kNBLarge - 2);
-      for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
-        const int k = H.large_nodes[q];
+  }
+  // The factorization's schedule.  Node k starts at tick tk_start[k] and,
+  // if large, runs one panel per tick until tick end[k]; an LDS or small
+  // front is factored by one launch at its start tick.  Children are
+  // numbered before their parents.
+  std::vector<int>& start = H.tk_start;
+  start.assign(N, 0);
+  std::vector<int> end(N, 0);
+  if (H.level_sched) {
+    // every node of level l starts at the level's first tick; the level's
+    // large fronts stay active for its largest panel count
+    std::vector<int> t0(nlev + 1, 0);
+    for (int l = 0; l < nlev; l++) t0[l + 1] = t0[l] + std::max(1, H.large_panels[l]);
+    for (int k = 0; k < N; k++) {
+      const int l = H.level[k];
+      start[k] = t0[l];
+      end[k] = is_large(k) ? t0[l] + H.large_panels[l] - 1 : t0[l];
+    }
+  } else {
+    for (int k = 0; k < N; k++) {
+      int t = 0;
+      for (int q = S.children_ptr[k]; q < S.children_ptr[k + 1]; q++)
+        t = std::max(t, end[S.children[q]] + 1);
+      start[k] = t;
+      end[k] = is_large(k) ? t + panels_of(k) - 1 : t;
+    }
+  }
+  int T = 0;
+  for (int k = 0; k < N; k++) T = std::max(T, end[k] + 1);
+  H.ntick = T;
+  // nodes by start tick (node order inside a tick); LDS fronts sorted by cap
+  auto bucket = [&](std::vector<int>& ptr, std::vector<int>& list, auto pick) {
+    ptr.assign(T + 1, 0);
+    for (int k = 0; k < N; k++)
+      if (pick(k)) ptr[start[k] + 1]++;
+    for (int t = 0; t < T; t++) ptr[t + 1] += ptr[t];
+    list.assign(ptr[T], 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int k = 0; k < N; k++)
+      if (pick(k)) list[fill[start[k]]++] = k;
+  };
+  bucket(H.tk_grp_ptr, H.tk_grp_nodes, [](int) { return true; });
+  bucket(H.tk_lds_ptr, H.tk_lds_nodes, [&](int k) { return H.cap[k] <= H.lds_front; });
+  bucket(H.tk_small_ptr, H.tk_small_nodes,
+         [&](int k) { return H.cap[k] > H.lds_front && !is_large(k); });
+  for (int t = 0; t < T; t++)
+    std::sort(H.tk_lds_nodes.begin() + H.tk_lds_ptr[t], H.tk_lds_nodes.begin() + H.tk_lds_ptr[t + 1],
+              [&](int a, int b) { return H.cap[a] < H.cap[b]; });
+  // assembly work items of the fronts assembled in HBM, by start tick
+  H.asm_node.clear();
+  H.asm_chunk.clear();
+  H.tk_asm_ptr.assign(T + 1, 0);
+  for (int t = 0; t < T; t++) {
+    for (int q = H.tk_grp_ptr[t]; q < H.tk_grp_ptr[t + 1]; q++) {
+      const int k = H.tk_grp_nodes[q];
+      if (H.cap[k] <= H.lds_front) continue;  // assembled inside k_front_lds
+      const int nch = (H.cap[k] + kAsmCols - 1) / kAsmCols;
+      for (int c = 0; c < nch; c++) {
+        H.asm_node.push_back(k);
+        H.asm_chunk.push_back(c);
+      }
+    }
+    H.tk_asm_ptr[t + 1] = (int)H.asm_node.size();
+  }
+  // active large fronts per tick and their update tiles
+  {
+    std::vector<std::vector<int>> act(T);
+    for (int k = 0; k < N; k++)
+      if (is_large(k))
+        for (int t = start[k]; t <= end[k]; t++) act[t].push_back(k);
+    H.tk_act_ptr.assign(T + 1, 0);
+    H.tk_act_nodes.clear();
+    H.tk_act_cap.assign(T, 0);
+    H.tk_act_root.assign(T, 0);
+    H.upd_ptr.assign(T + 1, 0);
+    H.upd_node.clear();
+    H.upd_tile.clear();
+    H.upd_base.clear();
+    for (int t = 0; t < T; t++) {
+      for (int k : act[t]) {
+        H.tk_act_nodes.push_back(k);
+        H.tk_act_cap[t] = std::max(H.tk_act_cap[t], H.cap[k]);
+        H.tk_act_root[t] |= H.mode[k] == 1;
+        const int base = (t - start[k]) * (kNBLarge - 2);
         const int nt = (H.cap[k] - base + 63) / 64;
         for (int tt = 0; tt < nt * (nt + 1) / 2; tt++) {
           H.upd_node.push_back(k);
           H.upd_tile.push_back(tt);
+          H.upd_base.push_back(base);
         }
       }
-      H.upd_ptr.push_back((int)H.upd_node.size());
+      H.tk_act_ptr[t + 1] = (int)H.tk_act_nodes.size();
+      H.upd_ptr[t + 1] = (int)H.upd_node.size();
     }
   }
-  H.upd_lvl[nlev] = (int)H.upd_ptr.size() - 1;
+  // panel scratch W (cap x kNB per front factored in HBM), used only by the
+  // factor kernels of the front's own ticks, all on the main stream: a
+  // region is reused by a front that starts after the region's last front
+  // ended (best fit by size, in start order)
+  {
+    struct Region {
+      long long off, size;
+      int until;
+    };
+    std::vector<Region> regions;
+    long long w = 0;
+    for (int q = 0; q < (int)H.tk_grp_nodes.size(); q++) {
+      const int k = H.tk_grp_nodes[q];
+      H.woff[k] = 0;
+      if (H.cap[k] <= H.lds_front) continue;
+      const long long need = (long long)H.cap[k] * kNB;
+      int best = -1;
+      for (int r = 0; r < (int)regions.size(); r++)
+        if (regions[r].until < start[k] && regions[r].size >= need &&
+            (best < 0 || regions[r].size < regions[best].size))
+          best = r;
+      if (best < 0) {
+        regions.push_back({w, need, 0});
+        w += need;
+        best = (int)regions.size() - 1;
+      }
+      regions[best].until = end[k];
+      H.woff[k] = regions[best].off;
+    }
+    H.w_total = std::max(w, 1LL);
+  }
   H.sv_ptr.assign(nlev + 1, 0);
   H.sv_tiny.assign(nlev, 0);
   H.sv_lt.assign(2 * (size_t)nlev, 0);
@@ -1752,7 +1850,7 @@ static void upload_static(Handle& H) {
   H.d_ioff.upload(H.ioff);
   H.d_woff.upload(H.woff);
   H.d_uoff.upload(H.uoff);
-  H.d_lvl_nodes.upload(H.lvl_nodes);
+  H.d_tk_grp_nodes.upload(H.tk_grp_nodes);
   H.d_asm_node.upload(H.asm_node);
   H.d_asm_chunk.upload(H.asm_chunk);
   H.d_rowp.upload(H.rowp);
@@ -1767,11 +1865,13 @@ static void upload_static(Handle& H) {
   H.d_need.alloc(N);
   H.d_cnt.alloc(5 * (size_t)N);  // [3N] inertia counters, [2N] zero / non-finite pivots
   H.d_ps.alloc(N);
-  H.d_small_nodes.upload(H.small_nodes);
-  H.d_lds_nodes.upload(H.lds_nodes);
+  H.d_tk_small_nodes.upload(H.tk_small_nodes);
+  H.d_tk_lds_nodes.upload(H.tk_lds_nodes);
+  H.d_tk_act_nodes.upload(H.tk_act_nodes);
   H.d_large_nodes.upload(H.large_nodes);
   H.d_upd_node.upload(H.upd_node);
   H.d_upd_tile.upload(H.upd_tile);
+  H.d_upd_base.upload(H.upd_base);
   H.d_sv_nodes.upload(H.sv_nodes);
   H.d_bg_node.upload(H.bg_node);
   H.d_bg_tile.upload(H.bg_tile);
@@ -1928,81 +2028,80 @@ static size_t lds_front_bytes(int c, size_t esize) {
 template <class T>
 static void run_factor_levels_t(Handle& H, hipStream_t st) {
   DevPlan P = dev_plan(H);
-  const int nlev = (int)H.lvl_ptr.size() - 1;
   bool joined = true;  // the side stream's work is ordered before st's next launch
-  for (int l = 0; l < nlev; l++) {
-    int n0 = H.lvl_ptr[l], n1 = H.lvl_ptr[l + 1];
-    if (!joined) {
-      HIPCHECK(hipStreamWaitEvent(st, H.ev_join, 0));
-      joined = true;
+  for (int t = 0; t < H.ntick; t++) {
+    const int n0 = H.tk_grp_ptr[t], n1 = H.tk_grp_ptr[t + 1];
+    const int d0 = H.tk_lds_ptr[t], d1 = H.tk_lds_ptr[t + 1];
+    const int a0 = H.tk_asm_ptr[t], a1 = H.tk_asm_ptr[t + 1];
+    const int s0 = H.tk_small_ptr[t], s1 = H.tk_small_ptr[t + 1];
+    const int g0 = H.tk_act_ptr[t], g1 = H.tk_act_ptr[t + 1];
+    if (n1 > n0) {
+      // the nodes starting now: their children ended at earlier ticks,
+      // LDS children on the side stream
+      if (!joined) {
+        HIPCHECK(hipStreamWaitEvent(st, H.ev_join, 0));
+        joined = true;
+      }
+      {
+        ProfScope ps_(H, st, KC_PREPARE);
+        hipLaunchKernelGGL(k_prepare, dim3((n1 - n0 + kTPB - 1) / kTPB), dim3(kTPB), 0, st,
+                           P, H.d_tk_grp_nodes.p + n0, n1 - n0);
+      }
+      // The LDS fronts (assembled in LDS) depend only on k_prepare and on
+      // their children; when the tick also has work on the main stream
+      // they run on the side stream beside it, and the next nodes to start
+      // wait for both.
+      const bool fork = d1 > d0 && (a1 > a0 || s1 > s0 || g1 > g0);
+      hipStream_t ss = st;
+      if (fork) {
+        HIPCHECK(hipEventRecord(H.ev_fork, st));
+        HIPCHECK(hipStreamWaitEvent(H.side, H.ev_fork, 0));
+        ss = H.side;
+      }
+      // LDS fronts in size buckets, so that small fronts get small LDS
+      // allocations and several workgroups per CU
+      for (int q0 = d0; q0 < d1;) {
+        const int c0 = H.cap[H.tk_lds_nodes[q0]];
+        int bucket = 32;
+        while (bucket < c0) bucket += 16;
+        int q1 = q0;
+        while (q1 < d1 && H.cap[H.tk_lds_nodes[q1]] <= bucket) q1++;
+        const size_t bytes = lds_front_bytes(bucket, sizeof(T));
+        ProfScope ps_(H, ss, KC_FRONT_LDS);
+        hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(q1 - q0), dim3(kTPB), bytes, ss, P,
+                           H.d_tk_lds_nodes.p + q0);
+        q0 = q1;
+      }
+      if (fork) {
+        HIPCHECK(hipEventRecord(H.ev_join, H.side));
+        joined = false;
+      }
+      if (a1 > a0) {
+        ProfScope ps_(H, st, KC_ASSEMBLE);
+        hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
+                           H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
+      }
+      if (s1 > s0) {
+        ProfScope ps_(H, st, KC_FACTOR);
+        hipLaunchKernelGGL(k_factor<T>, dim3(s1 - s0), dim3(kTPB), 0, st, P,
+                           H.d_tk_small_nodes.p + s0);
+      }
     }
-    {
-      ProfScope ps_(H, st, KC_PREPARE);
-      hipLaunchKernelGGL(k_prepare, dim3((n1 - n0 + kTPB - 1) / kTPB), dim3(kTPB), 0, st,
-                         P, H.d_lvl_nodes.p + n0, n1 - n0);
-    }
-    // The level's LDS fronts (assembled in LDS) depend only on k_prepare
-    // and on the levels below; when the level also has fronts assembled in
-    // HBM they run on the side stream, beside that assembly and those
-    // fronts' factorization, and the next level waits for both.
-    const bool fork = H.lds_ptr[l + 1] > H.lds_ptr[l] &&
-                      (H.large_ptr[l + 1] > H.large_ptr[l] || H.small_ptr[l + 1] > H.small_ptr[l]);
-    hipStream_t ss = st;
-    if (fork) {
-      HIPCHECK(hipEventRecord(H.ev_fork, st));
-      HIPCHECK(hipStreamWaitEvent(H.side, H.ev_fork, 0));
-      ss = H.side;
-    }
-    // LDS fronts in size buckets, so that small fronts get small LDS
-    // allocations and several workgroups per CU
-    for (int d0 = H.lds_ptr[l]; d0 < H.lds_ptr[l + 1];) {
-      const int c0 = H.cap[H.lds_nodes[d0]];
-      int bucket = 32;
-      while (bucket < c0) bucket += 16;
-      int d1 = d0;
-      while (d1 < H.lds_ptr[l + 1] && H.cap[H.lds_nodes[d1]] <= bucket) d1++;
-      const size_t bytes = lds_front_bytes(bucket, sizeof(T));
-      ProfScope ps_(H, ss, KC_FRONT_LDS);
-      hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(d1 - d0), dim3(kTPB), bytes, ss, P,
-                         H.d_lds_nodes.p + d0);
-      d0 = d1;
-    }
-    if (fork) {
-      HIPCHECK(hipEventRecord(H.ev_join, H.side));
-      joined = false;
-    }
-    int a0 = H.asm_lvl_ptr[l], a1 = H.asm_lvl_ptr[l + 1];
-    if (a1 > a0) {
-      ProfScope ps_(H, st, KC_ASSEMBLE);
-      hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
-                         H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
-    }
-    const int s0 = H.small_ptr[l], s1 = H.small_ptr[l + 1];
-    if (s1 > s0) {
-      ProfScope ps_(H, st, KC_FACTOR);
-      hipLaunchKernelGGL(k_factor<T>, dim3(s1 - s0), dim3(kTPB), 0, st, P,
-                         H.d_small_nodes.p + s0);
-    }
-    const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
-    int maxcap_large = 0;
-    bool root_large = false;  // a root front (Bunch-Kaufman tests) in the level
-    for (int q = g0; q < g1; q++) {
-      maxcap_large = std::max(maxcap_large, H.cap[H.large_nodes[q]]);
-      root_large = root_large || H.mode[H.large_nodes[q]] == 1;
-    }
-    for (int t = 0; t < H.large_panels[l]; t++) {
+    if (g1 > g0) {
+      // one panel of every large front active at this tick, then the
+      // trailing update of those panels
       {
         ProfScope ps_(H, st, KC_PANEL, true);
-        launch_large_panel((const T*)nullptr, root_large, maxcap_large, dim3(g1 - g0), st,
-                           ps_.start(), ps_.stop(), P, H.d_large_nodes.p + g0);
+        launch_large_panel((const T*)nullptr, H.tk_act_root[t] != 0, H.tk_act_cap[t],
+                           dim3(g1 - g0), st, ps_.start(), ps_.stop(), P,
+                           H.d_tk_act_nodes.p + g0);
       }
-      const int slot = H.upd_lvl[l] + t;
-      const int u0 = H.upd_ptr[slot], u1 = H.upd_ptr[slot + 1];
+      const int u0 = H.upd_ptr[t], u1 = H.upd_ptr[t + 1];
       if (u1 > u0) {
         ProfScope ps_(H, st, KC_UPDATE, true);
         hipExtLaunchKernelGGL(k_update<T>, dim3(u1 - u0), dim3(kTPB), 0, st, ps_.start(),
                               ps_.stop(), 0, P, H.d_upd_node.p + u0, H.d_upd_tile.p + u0,
-                              t * (kNBLarge - 2));
+                              H.d_upd_base.p + u0);
       }
     }
   }
@@ -2417,6 +2516,7 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   H.opt.perm = nullptr;
   H.opt.vals_hint = nullptr;
   build_maps(H);
+  if (const char* e = std::getenv("KKT_LDL_LEVEL_SCHEDULE")) H.level_sched = std::atoi(e) != 0;
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));
   H.stats.n = n;
@@ -2453,6 +2553,9 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
     long long lp = 0;
     for (int l = 0; l < nlev; l++) lp += H.large_panels[l];
     H.stats.level_panel_sum = lp;
+    long long pl = 0;
+    for (int t = 0; t < H.ntick; t++) pl += H.tk_act_ptr[t + 1] > H.tk_act_ptr[t];
+    H.stats.panel_launches = pl;
   }
   H.stats.t_analyze_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

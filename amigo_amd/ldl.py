@@ -119,6 +119,7 @@ class Stats(ctypes.Structure):
         ("num_nonfinite_pivots", ctypes.c_int),
         ("crit_path_pivots", ctypes.c_longlong),
         ("level_panel_sum", ctypes.c_longlong),
+        ("panel_launches", ctypes.c_longlong),
     ]
 
     def as_dict(self):

@@ -85,6 +85,10 @@ typedef struct {
                                * leaf-to-root path of the assembly tree     */
   long long level_panel_sum;  /* sum over tree levels of the largest number
                                * of 16-column panels of a front of the level */
+  long long panel_launches;   /* panel launches of the factorization's
+                               * schedule: a front starts once its children
+                               * are factored, so this is close to the
+                               * panels on the tree's longest path          */
 } kkt_ldl_stats;
 
 /* Defaults matching AmigoSolver (amigo/algorithm/solvers/amigo_solver.py:15-25). */
