@@ -552,7 +552,7 @@ __device__ void panel_factor(T* F, int ld, int m, int* vars, int* pre,
 // F[k:m, k:m] (lower) -= F[k:m, kstart:kstart+kw] * W[k:m, 0:kw]^T, one
 // workgroup, 4x4 register tiles.
 template <int NT, class T>
-__device__ void trailing_update_wg(T* F, int ld, int m, const double* W,
+__device__ __attribute__((always_inline)) void trailing_update_wg(T* F, int ld, int m, const double* W,
                                    int ldw, int kstart, int k, int kw) {
   const int nt = m - k;
   if (nt <= 0 || kw == 0) return;

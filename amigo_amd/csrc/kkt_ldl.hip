@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
 // once.  LDS layout: F (m x m, ld m) then W (m x kNB, ld m).
 // --------------------------------------------------------------------------
 template <int NT, class T>
-__global__ void __launch_bounds__(NT) k_front_lds(DevPlan P, const int* nodes) {
+__global__ void __launch_bounds__(NT, 4) k_front_lds(DevPlan P, const int* nodes) {
   extern __shared__ double lds[];
   __shared__ FrontShared<NT> sh;
   __shared__ ChildView<T> s_cv[kChildCache];
@@ -1415,6 +1415,11 @@ struct Handle {
   // the tile grid, from the front's panel index at t).
   int ntick = 0;
   std::vector<int> tk_start;  // per node: start tick
+  // side-stream event slot per tick (-1: no LDS fronts), and per tick the
+  // latest slot its starting nodes wait for (-1: none)
+  std::vector<int> tk_lds_ev, tk_wait;
+  int n_lds_ev = 0;
+  std::vector<hipEvent_t> lds_ev;
   std::vector<int> tk_grp_ptr, tk_grp_nodes, tk_lds_ptr, tk_lds_nodes, tk_small_ptr,
       tk_small_nodes, tk_asm_ptr, tk_act_ptr, tk_act_nodes, tk_act_cap, tk_act_root;
   std::vector<int> upd_ptr, upd_node, upd_tile, upd_base;
@@ -1453,11 +1458,18 @@ struct Handle {
   std::vector<int> h_npiv, h_ndel, h_cnt, h_stat;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   float last_factor_ms = 0.f, last_solve_ms = 0.f;
-  // second stream (default priority) for the small fronts of a level, which
-  // run beside its large-front chain; fork and join events
+  // second stream (default priority) for the LDS fronts, which run beside
+  // the panel chain: ev_fork orders them after their k_prepare, lds_ev[]
+  // (one per tick with LDS fronts) orders their parents after them
   hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr;
 };
+
+// LDS bytes of k_front_lds for a front of dimension c.
+static size_t lds_front_bytes(int c, size_t esize) {
+  return sizeof(double) * ((size_t)c * kNB + 2 * (size_t)c) + esize * (size_t)c * c +
+         3 * sizeof(int) * (size_t)c + 16;
+}
 
 static void build_plan(Handle& H, const std::vector<int>& slack) {
   const SymbolicResult& S = H.S;
@@ -1562,6 +1574,11 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
       end[k] = is_large(k) ? t0[l] + H.large_panels[l] - 1 : t0[l];
     }
   } else {
+    // a node starts at the tick after its last child ends; a large front
+    // ends after its panels, an LDS or small front at its start tick (an
+    // LDS front runs on the side stream; its parent's k_prepare waits for it
+    // by an event).  Starting LDS-front parents later, by a cost model of
+    // the side stream, was measured slower.
     for (int k = 0; k < N; k++) {
       int t = 0;
       for (int q = S.children_ptr[k]; q < S.children_ptr[k + 1]; q++)
@@ -1638,6 +1655,33 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
       H.upd_ptr[t + 1] = (int)H.upd_node.size();
     }
   }
+  // Side-stream events: one per tick that launches LDS fronts, recorded
+  // after them, in tick order; the nodes starting at tick t wait (before
+  // their k_prepare, which reads the children's results) for the event of
+  // the latest tick holding one of their LDS children.  The side stream runs
+  // in order, so that event covers every earlier LDS front as well.
+  // A tick whose LDS fronts are its only work runs them on the main stream
+  // instead: a hop between streams costs more than it hides there (measured
+  // about 18 us per dependent hop on a chain of LDS fronts).
+  H.tk_lds_ev.assign(T, -1);
+  H.n_lds_ev = 0;
+  for (int t = 0; t < T; t++) {
+    const bool lds = H.tk_lds_ptr[t + 1] > H.tk_lds_ptr[t];
+    const bool other = H.tk_small_ptr[t + 1] > H.tk_small_ptr[t] ||
+                       H.tk_lds_ptr[t + 1] - H.tk_lds_ptr[t] < H.tk_grp_ptr[t + 1] - H.tk_grp_ptr[t] ||
+                       H.tk_act_ptr[t + 1] > H.tk_act_ptr[t];
+    if (lds && other) H.tk_lds_ev[t] = H.n_lds_ev++;
+  }
+  H.tk_wait.assign(T, -1);
+  for (int t = 0; t < T; t++)
+    for (int q = H.tk_grp_ptr[t]; q < H.tk_grp_ptr[t + 1]; q++) {
+      const int k = H.tk_grp_nodes[q];
+      for (int c = S.children_ptr[k]; c < S.children_ptr[k + 1]; c++) {
+        const int ch = S.children[c];
+        if (H.cap[ch] <= H.lds_front)  // -1: that tick ran its LDS fronts on this stream
+          H.tk_wait[t] = std::max(H.tk_wait[t], H.tk_lds_ev[start[ch]]);
+      }
+    }
   // panel scratch W (cap x kNB per front factored in HBM), used only by the
   // factor kernels of the front's own ticks, all on the main stream: a
   // region is reused by a front that starts after the region's last front
@@ -2019,16 +2063,15 @@ static void prof_collect(Handle& H) {
   H.prof_cls.clear();
 }
 
-// LDS bytes of k_front_lds for a front of dimension c.
-static size_t lds_front_bytes(int c, size_t esize) {
-  return sizeof(double) * ((size_t)c * kNB + 2 * (size_t)c) + esize * (size_t)c * c +
-         3 * sizeof(int) * (size_t)c + 16;
-}
-
 template <class T>
 static void run_factor_levels_t(Handle& H, hipStream_t st) {
   DevPlan P = dev_plan(H);
-  bool joined = true;  // the side stream's work is ordered before st's next launch
+  while ((int)H.lds_ev.size() < H.n_lds_ev) {
+    hipEvent_t e;
+    HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    H.lds_ev.push_back(e);
+  }
+  int waited = -1;  // latest side-stream event the main stream has waited for
   for (int t = 0; t < H.ntick; t++) {
     const int n0 = H.tk_grp_ptr[t], n1 = H.tk_grp_ptr[t + 1];
     const int d0 = H.tk_lds_ptr[t], d1 = H.tk_lds_ptr[t + 1];
@@ -2036,11 +2079,11 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
     const int s0 = H.tk_small_ptr[t], s1 = H.tk_small_ptr[t + 1];
     const int g0 = H.tk_act_ptr[t], g1 = H.tk_act_ptr[t + 1];
     if (n1 > n0) {
-      // the nodes starting now: their children ended at earlier ticks,
-      // LDS children on the side stream
-      if (!joined) {
-        HIPCHECK(hipStreamWaitEvent(st, H.ev_join, 0));
-        joined = true;
+      // the nodes starting now: their large children ended at earlier ticks
+      // on this stream; their LDS children ran on the side stream
+      if (H.tk_wait[t] > waited) {
+        waited = H.tk_wait[t];
+        HIPCHECK(hipStreamWaitEvent(st, H.lds_ev[waited], 0));
       }
       {
         ProfScope ps_(H, st, KC_PREPARE);
@@ -2048,33 +2091,29 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
                            P, H.d_tk_grp_nodes.p + n0, n1 - n0);
       }
       // The LDS fronts (assembled in LDS) depend only on k_prepare and on
-      // their children; when the tick also has work on the main stream
-      // they run on the side stream beside it, and the next nodes to start
-      // wait for both.
-      const bool fork = d1 > d0 && (a1 > a0 || s1 > s0 || g1 > g0);
-      hipStream_t ss = st;
-      if (fork) {
-        HIPCHECK(hipEventRecord(H.ev_fork, st));
-        HIPCHECK(hipStreamWaitEvent(H.side, H.ev_fork, 0));
-        ss = H.side;
-      }
-      // LDS fronts in size buckets, so that small fronts get small LDS
-      // allocations and several workgroups per CU
-      for (int q0 = d0; q0 < d1;) {
-        const int c0 = H.cap[H.tk_lds_nodes[q0]];
-        int bucket = 32;
-        while (bucket < c0) bucket += 16;
-        int q1 = q0;
-        while (q1 < d1 && H.cap[H.tk_lds_nodes[q1]] <= bucket) q1++;
-        const size_t bytes = lds_front_bytes(bucket, sizeof(T));
-        ProfScope ps_(H, ss, KC_FRONT_LDS);
-        hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(q1 - q0), dim3(kTPB), bytes, ss, P,
-                           H.d_tk_lds_nodes.p + q0);
-        q0 = q1;
-      }
-      if (fork) {
-        HIPCHECK(hipEventRecord(H.ev_join, H.side));
-        joined = false;
+      // their children; they run on the side stream beside the tick's other
+      // work, in size buckets, so that small fronts get small LDS
+      // allocations and several workgroups per CU.
+      if (d1 > d0) {
+        const int ev = H.tk_lds_ev[t];
+        hipStream_t ss = ev >= 0 ? H.side : st;
+        if (ev >= 0) {
+          HIPCHECK(hipEventRecord(H.ev_fork, st));
+          HIPCHECK(hipStreamWaitEvent(H.side, H.ev_fork, 0));
+        }
+        for (int q0 = d0; q0 < d1;) {
+          const int c0 = H.cap[H.tk_lds_nodes[q0]];
+          int bucket = 32;
+          while (bucket < c0) bucket += 16;
+          int q1 = q0;
+          while (q1 < d1 && H.cap[H.tk_lds_nodes[q1]] <= bucket) q1++;
+          const size_t bytes = lds_front_bytes(bucket, sizeof(T));
+          ProfScope ps_(H, ss, KC_FRONT_LDS);
+          hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(q1 - q0), dim3(kTPB), bytes, ss, P,
+                             H.d_tk_lds_nodes.p + q0);
+          q0 = q1;
+        }
+        if (ev >= 0) HIPCHECK(hipEventRecord(H.lds_ev[ev], H.side));
       }
       if (a1 > a0) {
         ProfScope ps_(H, st, KC_ASSEMBLE);
@@ -2105,7 +2144,9 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       }
     }
   }
-  if (!joined) HIPCHECK(hipStreamWaitEvent(st, H.ev_join, 0));
+  // every LDS front is done before the factorization's last kernels
+  if (H.n_lds_ev > 0 && waited < H.n_lds_ev - 1)
+    HIPCHECK(hipStreamWaitEvent(st, H.lds_ev[H.n_lds_ev - 1], 0));
   if (!H.inv_node.empty()) {
     ProfScope ps_(H, st, KC_INVERT);
     hipLaunchKernelGGL(k_panel_inv<T>, dim3((unsigned)H.inv_node.size()), dim3(kWave), 0, st, P,
@@ -2145,7 +2186,6 @@ static void ensure_device(Handle& H) {
   HIPCHECK(hipEventCreate(&H.ev3));
   HIPCHECK(hipStreamCreateWithFlags(&H.side, hipStreamNonBlocking));
   HIPCHECK(hipEventCreateWithFlags(&H.ev_fork, hipEventDisableTiming));
-  HIPCHECK(hipEventCreateWithFlags(&H.ev_join, hipEventDisableTiming));
   H.uploaded = true;
 }
 
@@ -2849,7 +2889,7 @@ void kkt_ldl_destroy(kkt_ldl_t h) {
   if (H.ev2) (void)hipEventDestroy(H.ev2);
   if (H.ev3) (void)hipEventDestroy(H.ev3);
   if (H.ev_fork) (void)hipEventDestroy(H.ev_fork);
-  if (H.ev_join) (void)hipEventDestroy(H.ev_join);
+  for (auto e : H.lds_ev) (void)hipEventDestroy(e);
   if (H.side) (void)hipStreamDestroy(H.side);
   delete h;
 }
