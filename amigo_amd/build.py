@@ -2,7 +2,7 @@
 
     python -m amigo_amd.build [--incremental]
 
-Compiles amigo_amd/csrc/{symbolic.cpp, nd_ordering.cpp, kkt_ldl.hip, kkt_panel.hip (once per
+Compiles amigo_amd/csrc/{symbolic.cpp, nd_ordering.cpp, kkt_ldl.hip, kkt_front.hip, kkt_panel.hip (once per
 front element type), assembly.hip, ipm.hip} with hipcc, in parallel, into
 amigo_amd/libkktldl.so.  Every HIP translation unit is built with packed FP32
 arithmetic disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
@@ -36,6 +36,7 @@ UNITS = (
     ("symbolic.cpp", "symbolic.o", []),
     ("nd_ordering.cpp", "nd_ordering.o", []),
     ("kkt_ldl.hip", "kkt_ldl.o", ["-x", "hip"]),
+    ("kkt_front.hip", "kkt_front.o", ["-x", "hip"]),
     ("kkt_panel.hip", "kkt_panel_f32.o", ["-x", "hip", "-DKKT_PANEL_T=float"]),
     ("kkt_panel.hip", "kkt_panel_f64.o", ["-x", "hip", "-DKKT_PANEL_T=double"]),
     ("assembly.hip", "assembly.o", ["-x", "hip"]),

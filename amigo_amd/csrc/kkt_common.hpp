@@ -66,6 +66,8 @@ struct DevPlan {
   const int* node_to_var;
   const int* contrib_rows;
   const int* invnom;   // for each child: parent's nominal position -> child contrib index or -1
+  const int* fwdnom;   // per contribution row of a child (contrib_rows order): the
+                       // parent's nominal position of that variable
   const int* am_ptr;   // per own column (global index g in node_to_var order)
   const int* am_src;   // CSR index into vals
   const int* am_row;   // nominal row in the front
@@ -689,6 +691,15 @@ bool launch_large_panel(const float*, bool bkc, int maxcap, dim3 grid, hipStream
                         hipEvent_t e0, hipEvent_t e1, const DevPlan& P, const int* nodes);
 bool launch_large_panel(const double*, bool bkc, int maxcap, dim3 grid, hipStream_t st,
                         hipEvent_t e0, hipEvent_t e1, const DevPlan& P, const int* nodes);
+
+// Fronts held whole in LDS (kkt_front.hip), threshold / Bunch-Kaufman modes:
+// `count` fronts of at most `cap` rows, one workgroup each (chain = false),
+// or one workgroup factoring them in list order (chain = true).
+void launch_front_rl(const float*, int cap, int count, bool chain, hipStream_t st,
+                     const DevPlan& P, const int* nodes);
+void launch_front_rl(const double*, int cap, int count, bool chain, hipStream_t st,
+                     const DevPlan& P, const int* nodes);
+int rl_max_front();
 
 }  // namespace kktldl
 

@@ -1389,13 +1389,13 @@ struct Handle {
   // plan (host)
   std::vector<int> ns, cs, cap, mode, level, lvl_ptr, lvl_nodes;
   std::vector<long long> foff, ioff, woff, uoff, invoff;
-  std::vector<int> invnom, am_ptr, am_src, am_row, am_col;
+  std::vector<int> invnom, fwdnom, am_ptr, am_src, am_row, am_col;
   std::vector<int> asm_node, asm_chunk;
   long long f_total = 0, i_total = 0, w_total = 0, u_total = 0;
   int max_cap = 0;
   // device
   DBuf<int> d_ns, d_cs, d_cap, d_vptr, d_cptr, d_chptr, d_ch, d_mode, d_n2v,
-      d_crows, d_invnom, d_amptr, d_amsrc, d_amrow, d_amcol, d_asm_node,
+      d_crows, d_invnom, d_fwdnom, d_amptr, d_amsrc, d_amrow, d_amcol, d_asm_node,
       d_asm_chunk, d_rowp, d_cols;
   DBuf<long long> d_invoff, d_foff, d_ioff, d_woff, d_uoff;
   DBuf<int> d_din, d_m, d_npiv, d_ndel, d_doff, d_stat, d_need, d_cnt;
@@ -1418,6 +1418,10 @@ struct Handle {
   // side-stream event slot per tick (-1: no LDS fronts), and per tick the
   // latest slot its starting nodes wait for (-1: none)
   std::vector<int> tk_lds_ev, tk_wait;
+  std::vector<int> tk_chain_end;
+  // LDS fronts take the right-looking kernels of kkt_front.hip (every mode
+  // but Cholesky, which keeps k_front_lds)
+  bool rl_fronts = true;
   int n_lds_ev = 0;
   std::vector<hipEvent_t> lds_ev;
   std::vector<int> tk_grp_ptr, tk_grp_nodes, tk_lds_ptr, tk_lds_nodes, tk_small_ptr,
@@ -1682,6 +1686,30 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
           H.tk_wait[t] = std::max(H.tk_wait[t], H.tk_lds_ev[start[ch]]);
       }
     }
+  // Chains: runs of consecutive ticks whose only work is one LDS front
+  // (threshold or Bunch-Kaufman mode) are factored by one workgroup in one
+  // launch (k_front_chain); tk_chain_end[t] is the run's last tick at its
+  // first tick t, else -1.  Every front of a run has its children in
+  // earlier ticks, so the list order is an elimination order.
+  H.tk_chain_end.assign(T, -1);
+  if (H.rl_fronts) {
+    auto chain_tick = [&](int t) {
+      if (H.tk_grp_ptr[t + 1] - H.tk_grp_ptr[t] != 1) return false;
+      if (H.tk_act_ptr[t + 1] > H.tk_act_ptr[t]) return false;
+      const int k = H.tk_grp_nodes[H.tk_grp_ptr[t]];
+      return H.cap[k] <= H.lds_front && H.mode[k] != 2;
+    };
+    for (int t = 0; t < T;) {
+      if (!chain_tick(t)) {
+        t++;
+        continue;
+      }
+      int e = t;
+      while (e + 1 < T && chain_tick(e + 1)) e++;
+      H.tk_chain_end[t] = e;
+      t = e + 1;
+    }
+  }
   // panel scratch W (cap x kNB per front factored in HBM), used only by the
   // factor kernels of the front's own ticks, all on the main stream: a
   // region is reused by a front that starts after the region's last front
@@ -1839,6 +1867,23 @@ static void build_maps(Handle& H) {
     for (int q = S.contrib_ptr[c]; q < S.contrib_ptr[c + 1]; q++)
       mark[S.contrib_rows[q]] = -1;
   }
+  // forward maps: for every contribution row of a child, the parent's
+  // nominal position of that variable (own columns, then contribution rows)
+  H.fwdnom.assign((size_t)S.contrib_ptr[N] + 1, -1);
+  for (int c = 0; c < N; c++) {
+    int p = S.node_parent[c];
+    if (p < 0) continue;
+    const int nsp = S.node_ptr[p + 1] - S.node_ptr[p];
+    for (int q = S.node_ptr[p]; q < S.node_ptr[p + 1]; q++) mark[S.node_to_var[q]] = q - S.node_ptr[p];
+    for (int q = S.contrib_ptr[p]; q < S.contrib_ptr[p + 1]; q++)
+      mark[S.contrib_rows[q]] = nsp + (q - S.contrib_ptr[p]);
+    for (int q = S.contrib_ptr[c]; q < S.contrib_ptr[c + 1]; q++) {
+      H.fwdnom[q] = mark[S.contrib_rows[q]];
+      if (H.fwdnom[q] < 0) throw Error(-2, "contribution row outside the parent's structure");
+    }
+    for (int q = S.node_ptr[p]; q < S.node_ptr[p + 1]; q++) mark[S.node_to_var[q]] = -1;
+    for (int q = S.contrib_ptr[p]; q < S.contrib_ptr[p + 1]; q++) mark[S.contrib_rows[q]] = -1;
+  }
   // original entries per own column: entries (i, var) with permuted index
   // of i >= permuted index of var, placed at the front's nominal row.
   H.am_ptr.assign(n + 1, 0);
@@ -1885,6 +1930,7 @@ static void upload_static(Handle& H) {
   H.d_n2v.upload(S.node_to_var);
   H.d_crows.upload(S.contrib_rows);
   H.d_invnom.upload(H.invnom);
+  H.d_fwdnom.upload(H.fwdnom);
   H.d_invoff.upload(H.invoff);
   H.d_amptr.upload(H.am_ptr);
   H.d_amsrc.upload(H.am_src);
@@ -1960,6 +2006,7 @@ static DevPlan dev_plan(Handle& H) {
   P.node_to_var = H.d_n2v.p;
   P.contrib_rows = H.d_crows.p;
   P.invnom = H.d_invnom.p;
+  P.fwdnom = H.d_fwdnom.p;
   P.am_ptr = H.d_amptr.p;
   P.am_src = H.d_amsrc.p;
   P.am_row = H.d_amrow.p;
@@ -2073,6 +2120,26 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
   }
   int waited = -1;  // latest side-stream event the main stream has waited for
   for (int t = 0; t < H.ntick; t++) {
+    if (H.tk_chain_end[t] >= 0) {
+      // a chain of single-front ticks: one workgroup, one launch
+      const int e = H.tk_chain_end[t];
+      int wait = -1, cap = 0;
+      for (int u = t; u <= e; u++) {
+        wait = std::max(wait, H.tk_wait[u]);
+        cap = std::max(cap, H.cap[H.tk_grp_nodes[H.tk_grp_ptr[u]]]);
+      }
+      if (wait > waited) {
+        waited = wait;
+        HIPCHECK(hipStreamWaitEvent(st, H.lds_ev[waited], 0));
+      }
+      {
+        ProfScope ps_(H, st, KC_FRONT_LDS);
+        launch_front_rl((const T*)nullptr, cap, e - t + 1, true, st, P,
+                        H.d_tk_grp_nodes.p + H.tk_grp_ptr[t]);
+      }
+      t = e;
+      continue;
+    }
     const int n0 = H.tk_grp_ptr[t], n1 = H.tk_grp_ptr[t + 1];
     const int d0 = H.tk_lds_ptr[t], d1 = H.tk_lds_ptr[t + 1];
     const int a0 = H.tk_asm_ptr[t], a1 = H.tk_asm_ptr[t + 1];
@@ -2107,10 +2174,15 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
           while (bucket < c0) bucket += 16;
           int q1 = q0;
           while (q1 < d1 && H.cap[H.tk_lds_nodes[q1]] <= bucket) q1++;
-          const size_t bytes = lds_front_bytes(bucket, sizeof(T));
           ProfScope ps_(H, ss, KC_FRONT_LDS);
-          hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(q1 - q0), dim3(kTPB), bytes, ss, P,
-                             H.d_tk_lds_nodes.p + q0);
+          if (H.rl_fronts) {
+            launch_front_rl((const T*)nullptr, std::min(bucket, H.lds_front), q1 - q0, false, ss,
+                            P, H.d_tk_lds_nodes.p + q0);
+          } else {
+            const size_t bytes = lds_front_bytes(bucket, sizeof(T));
+            hipLaunchKernelGGL((k_front_lds<kTPB, T>), dim3(q1 - q0), dim3(kTPB), bytes, ss, P,
+                               H.d_tk_lds_nodes.p + q0);
+          }
           q0 = q1;
         }
         if (ev >= 0) HIPCHECK(hipEventRecord(H.lds_ev[ev], H.side));
@@ -2557,6 +2629,11 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   H.opt.vals_hint = nullptr;
   build_maps(H);
   if (const char* e = std::getenv("KKT_LDL_LEVEL_SCHEDULE")) H.level_sched = std::atoi(e) != 0;
+  // KKT_LDL_RL_FRONTS=0 keeps the blocked LDS front kernel (k_front_lds)
+  // and one launch per tick for every mode
+  H.rl_fronts = H.opt.solver_type != KKT_LDL_CHOLESKY;
+  if (const char* e = std::getenv("KKT_LDL_RL_FRONTS")) H.rl_fronts &= std::atoi(e) != 0;
+  H.lds_front = std::min(H.lds_front, rl_max_front());
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));
   H.stats.n = n;

@@ -1,0 +1,91 @@
+"""Fronts held whole in LDS: the right-looking kernels (k_front_rl, one
+workgroup per front of a tick; k_front_chain, one workgroup walking a run
+of single-front ticks) against the oracle node by node, and against the
+blocked LDS kernel (k_front_lds, selected with KKT_LDL_RL_FRONTS=0 when the
+handle is created) on the same plan.
+
+The cart-pole KKT under AMD is one chain of 2,858 fronts of 68 rows (every
+tick holds one LDS front: a single k_front_chain launch); under nested
+dissection its fronts are LDS fronts many per tick.  Bar: per-node pivot,
+delay and 2x2 counts below the root identical to the oracle's, inertia
+identical, refined residual <= 1e-10.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from amigo_amd import workloads as W
+from amigo_amd.ldl import FP32_REFINE, FP64, CSRMat, OrderingType, SparseLDL
+from oracle.pyoracle import AMD, GIVEN, OracleLDL
+
+pytestmark = pytest.mark.gpu
+
+
+def _relres(K, x, b):
+    return np.linalg.norm(K @ x - b) / np.linalg.norm(b)
+
+
+def _node_parity(s, o):
+    a, b = s.node_info(), o.node_info()
+    nonroot = b["root"] == 0
+    assert np.array_equal(a["npiv"][nonroot], b["npiv"][nonroot])
+    assert np.array_equal(a["ndel"][nonroot], b["ndel"][nonroot])
+    assert np.array_equal(a["n2x2"][nonroot], b["n2x2"][nonroot])
+
+
+@pytest.mark.parametrize("order", ["amd", "nd"])
+@pytest.mark.parametrize("prec", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
+def test_cartpole_full_size_node_parity(order, prec):
+    print("cartpole", order, prec, flush=True)
+    P = W.cartpole()
+    ot = OrderingType.AMD if order == "amd" else OrderingType.NESTED_DISSECTION
+    s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals), order=ot, precision=prec,
+                  max_refine=30)
+    assert s.factor(P.diag) == 0
+    st = s.stats()
+    if order == "amd":
+        assert st["num_levels"] == st["num_nodes"]  # one front per level: a chain
+    if order == "amd":
+        o = OracleLDL(P.n, P.rowp, P.cols, order=AMD)
+        assert np.array_equal(s.get_perm()[0], o.perm()[0])
+    else:
+        o = OracleLDL(P.n, P.rowp, P.cols, order=GIVEN, perm=s.get_perm()[0])
+    o.factor(P.vals, P.diag)
+    assert s.get_inertia() == o.inertia()
+    _node_parity(s, o)
+    x = P.rhs.copy()
+    s.solve(x, refine=True)
+    assert _relres(P.matrix(), x, P.rhs) <= 1e-10
+
+
+@pytest.mark.parametrize("name,kw,order", [("cartpole", dict(N=2000), OrderingType.AMD),
+                                           ("poisson2d", dict(N=64),
+                                            OrderingType.NESTED_DISSECTION),
+                                           ("rosenbrock", {}, OrderingType.AMD)])
+@pytest.mark.parametrize("prec", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
+def test_right_looking_matches_blocked_lds_kernel(name, kw, order, prec):
+    """Same plan, same decisions: the right-looking kernels and the blocked
+    k_front_lds give the same per-node pivot outcome and inertia, and
+    solutions that agree to rounding."""
+    print("rl vs blocked", name, prec, flush=True)
+    P = W.GENERATORS[name](**kw)
+    out = []
+    for flag in ("1", "0"):
+        os.environ["KKT_LDL_RL_FRONTS"] = flag
+        try:
+            s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals), order=order,
+                          precision=prec, max_refine=30)
+        finally:
+            del os.environ["KKT_LDL_RL_FRONTS"]
+        assert s.factor(P.diag) == 0
+        x = P.rhs.copy()
+        s.solve(x, refine=True)
+        out.append((s.get_inertia(), s.node_info(), x))
+    (i1, n1, x1), (i0, n0, x0) = out
+    assert i1 == i0
+    for k in ("npiv", "ndel", "n2x2"):
+        assert np.array_equal(n1[k], n0[k]), k
+    assert np.abs(x1 - x0).max() <= 1e-8 * np.abs(x0).max()
+    assert _relres(P.matrix(), x1, P.rhs) <= 1e-10
