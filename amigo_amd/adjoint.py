@@ -59,23 +59,51 @@ def kkt_solve(solver, b, refine=True):
 
 
 def post_optimality_derivatives(solver, out_wrt_x, grad_wrt_p, out_wrt_p=None,
-                                refine=True):
+                                refine=True, as_numpy=True):
     """dfdp (nout x np) by the adjoint method, all outputs in one solve.
 
     out_wrt_x:  (nout, n) d out / dx at the optimum (dense array or tensor)
-    grad_wrt_p: (n, np) d g / dp (scipy sparse, dense array or tensor)
+    grad_wrt_p: (n, np) d g / dp: scipy sparse (multiplied on the host), a
+                dense array or tensor, or a ``DeviceJacobian`` (sparse, kept
+                in HBM and multiplied on the GPU)
     out_wrt_p:  (nout, np) explicit d out / dp, or None
+    as_numpy:   return a numpy array (host copy); False returns the CUDA
+                tensor, so that a caller whose inputs already live in HBM
+                moves nothing over PCIe
     """
     A = _as_cuda(out_wrt_x)
     if A.dim() == 1:
         A = A[None, :]
     psi = _colmajor(-A.t())            # n x nout, column major
     solver.solve(psi, refine=refine)   # K psi = -d out / dx
-    if hasattr(grad_wrt_p, "tocsr"):   # scipy sparse: (d g/dp)^T psi on the host
+    if isinstance(grad_wrt_p, DeviceJacobian):
+        dfdp = grad_wrt_p.tmatmul(psi).t()
+    elif hasattr(grad_wrt_p, "tocsr"):   # scipy sparse: (d g/dp)^T psi on the host
         G = grad_wrt_p.tocsr()
         dfdp = torch.as_tensor((G.T @ psi.cpu().numpy()).T)
     else:
-        dfdp = (psi.t() @ _as_cuda(grad_wrt_p)).cpu()
+        dfdp = psi.t() @ _as_cuda(grad_wrt_p)
     if out_wrt_p is not None:
-        dfdp = dfdp + torch.as_tensor(np.asarray(out_wrt_p, dtype=np.float64))
-    return dfdp.numpy()
+        dfdp = dfdp + torch.as_tensor(np.asarray(out_wrt_p, dtype=np.float64),
+                                      device=dfdp.device)
+    return dfdp.cpu().numpy() if as_numpy else dfdp
+
+
+class DeviceJacobian:
+    """A sparse Jacobian d g / dp (n x np) held in HBM as the CSR matrix of
+    its transpose, for (d g / dp)^T psi on the GPU (one sparse x dense
+    product for every output at once)."""
+
+    def __init__(self, G):
+        Gt = G.T.tocsr()
+        Gt.sort_indices()
+        self.shape = G.shape
+        self.t = torch.sparse_csr_tensor(
+            torch.as_tensor(Gt.indptr, dtype=torch.int64, device="cuda"),
+            torch.as_tensor(Gt.indices, dtype=torch.int64, device="cuda"),
+            torch.as_tensor(Gt.data, dtype=torch.float64, device="cuda"),
+            size=(G.shape[1], G.shape[0]))
+
+    def tmatmul(self, psi):
+        """(d g / dp)^T psi for psi (n x k) on the device: (np x k)."""
+        return torch.sparse.mm(self.t, psi)

@@ -104,24 +104,71 @@ __global__ void k_prepare(DevPlan P, const int* nodes, int count) {
 // --------------------------------------------------------------------------
 constexpr int kChildCache = 64;
 
+// prep: the node's preparation (k_prepare's work) is done here instead, by
+// every workgroup of the node for itself (delay offsets of the children in
+// LDS) and written out by the workgroup of chunk 0; only for nodes with at
+// most kChildCache children.
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
-                                                   const int* wchunk) {
+                                                   const int* wchunk, int prep) {
   const int p = wnode[blockIdx.x];
   const int chunk = wchunk[blockIdx.x];
   const int tid = threadIdx.x;
-  if (P.stat[p]) return;
   const int c0 = P.chptr[p], nch = P.chptr[p + 1] - c0;
-  const int ns = P.ns[p], cs = P.cs[p], din = P.din[p], m = P.m[p];
-  const int j0 = chunk * kAsmCols;
-  if (j0 >= m) return;
-  const int j1 = min(m, j0 + kAsmCols);
+  const int ns = P.ns[p], cs = P.cs[p];
   __shared__ ChildView<T> s_cv[kChildCache];
   __shared__ int s_list[kChildCache];
   __shared__ int s_rj[kChildCache];
-  __shared__ int s_nl;
+  __shared__ int s_doff[kChildCache];
+  __shared__ int s_nl, s_fail, s_din;
+  if (prep) {
+    if (tid == 0) {
+      int din = 0, cfail = 0;
+      for (int q = 0; q < nch; q++) {
+        const int c = P.ch[c0 + q];
+        if (P.stat[c]) {
+          cfail = 1;
+          continue;
+        }
+        s_doff[q] = din;
+        if (chunk == 0) P.doff[c] = din;
+        din += P.ndel[c];
+      }
+      const int m = ns + din + cs;
+      const int fail = cfail || m > P.cap[p];
+      if (chunk == 0) {
+        P.din[p] = din;
+        P.m[p] = m;
+        P.npiv[p] = 0;
+        P.ndel[p] = 0;
+        P.stat[p] = fail;
+        if (fail) P.need[p] = cfail ? -1 : m;
+        PanelState st;
+        st.k = 0;
+        st.nc = P.mode[p] == 1 ? m : ns + din;
+        st.kstart = 0;
+        st.kw = 0;
+        st.npos = st.nneg = st.n2 = st.pad = 0;
+        P.ps[p] = st;
+      }
+      s_fail = fail;
+      s_din = din;
+    }
+    __syncthreads();
+    if (s_fail) return;
+  } else if (P.stat[p]) {
+    return;
+  }
+  const int din = prep ? s_din : P.din[p];
+  const int m = ns + din + cs;
+  const int j0 = chunk * kAsmCols;
+  if (j0 >= m) return;
+  const int j1 = min(m, j0 + kAsmCols);
   const int ncache = min(nch, kChildCache);
-  for (int t = tid; t < ncache; t += kTPB) s_cv[t] = load_child<T>(P, P.ch[c0 + t]);
+  for (int t = tid; t < ncache; t += kTPB) {
+    s_cv[t] = load_child<T>(P, P.ch[c0 + t]);
+    if (prep) s_cv[t].doff = s_doff[t];
+  }
   int* vars = P.vars + P.ioff[p];
   int* pre = P.pre + P.ioff[p];
   if (chunk == 0) {
@@ -131,7 +178,7 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
     for (int i = tid; i < m; i += kTPB) pre[i] = i;
     for (int q = 0; q < nch; q++) {
       const int c = P.ch[c0 + q];
-      const int nd = P.ndel[c], off = P.doff[c];
+      const int nd = P.ndel[c], off = prep ? s_doff[q] : P.doff[c];
       const int* cv = P.vars + P.ioff[c] + P.npiv[c];
       for (int t = tid; t < nd; t += kTPB) vars[ns + off + t] = cv[t];
     }
@@ -1419,6 +1466,7 @@ struct Handle {
   // latest slot its starting nodes wait for (-1: none)
   std::vector<int> tk_lds_ev, tk_wait;
   std::vector<int> tk_chain_end;
+  std::vector<int> tk_prep;  // per tick: k_prepare launched (else prepared inline)
   // LDS fronts take the right-looking kernels of kkt_front.hip (every mode
   // but Cholesky, which keeps k_front_lds)
   bool rl_fronts = true;
@@ -1686,6 +1734,19 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
           H.tk_wait[t] = std::max(H.tk_wait[t], H.tk_lds_ev[start[ch]]);
       }
     }
+  // k_prepare runs at a tick only where a node needs it before its factor
+  // kernel: Cholesky small fronts, LDS fronts of the blocked kernel, fronts
+  // with more children than k_assemble's LDS cache.  Otherwise k_assemble
+  // and the right-looking LDS kernels prepare their nodes themselves.
+  H.tk_prep.assign(T, 0);
+  for (int t = 0; t < T; t++) {
+    int need = !H.rl_fronts || H.tk_small_ptr[t + 1] > H.tk_small_ptr[t];
+    for (int q = H.tk_grp_ptr[t]; q < H.tk_grp_ptr[t + 1]; q++) {
+      const int k = H.tk_grp_nodes[q];
+      need |= S.children_ptr[k + 1] - S.children_ptr[k] > kChildCache;
+    }
+    H.tk_prep[t] = need;
+  }
   // Chains: runs of consecutive ticks whose only work is one LDS front
   // (threshold or Bunch-Kaufman mode) are factored by one workgroup in one
   // launch (k_front_chain); tk_chain_end[t] is the run's last tick at its
@@ -2152,7 +2213,7 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
         waited = H.tk_wait[t];
         HIPCHECK(hipStreamWaitEvent(st, H.lds_ev[waited], 0));
       }
-      {
+      if (H.tk_prep[t]) {
         ProfScope ps_(H, st, KC_PREPARE);
         hipLaunchKernelGGL(k_prepare, dim3((n1 - n0 + kTPB - 1) / kTPB), dim3(kTPB), 0, st,
                            P, H.d_tk_grp_nodes.p + n0, n1 - n0);
@@ -2190,7 +2251,7 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       if (a1 > a0) {
         ProfScope ps_(H, st, KC_ASSEMBLE);
         hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
-                           H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
+                           H.d_asm_node.p + a0, H.d_asm_chunk.p + a0, H.tk_prep[t] ? 0 : 1);
       }
       if (s1 > s0) {
         ProfScope ps_(H, st, KC_FACTOR);

@@ -587,26 +587,39 @@ def bench_c4(torch, reps=3):
     """Config 4: post-optimality derivatives dJ/dp on the C2 KKT for p =
     1,024 bilinear target coefficients and 64 seeded output functionals: the
     64 adjoint solves in one refined call on the stored FP64 factors plus
-    dfdp = (dg/dp)^T psi, median of reps."""
+    dfdp = (dg/dp)^T psi, with the output gradients and dg/dp resident in
+    HBM (the sparse product on the GPU), median of reps.  Also the same with
+    the inputs on the host and the product by scipy (PCIe and host work
+    included)."""
     from amigo_amd import workloads as W
-    from amigo_amd.adjoint import post_optimality_derivatives
+    from amigo_amd.adjoint import DeviceJacobian, post_optimality_derivatives
     from amigo_amd.ldl import FP64, CSRMat, OrderingType, SparseLDL
 
     P, S = W.poisson2d_sensitivity(N=256, ncoarse=32, nout=64)
     s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
                   order=OrderingType.NESTED_DISSECTION, precision=FP64)
     s.factor(P.diag)
-    post_optimality_derivatives(s, S["C"], S["dgdp"])
-    torch.cuda.synchronize()
-    tt = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        d = post_optimality_derivatives(s, S["C"], S["dgdp"])
+    C = torch.tensor(S["C"], device="cuda")
+    G = DeviceJacobian(S["dgdp"])
+    out = dict(workload="C2 KKT, p=1024 (32x32 bilinear), 64 functionals, FP64 factors")
+    for name, args_ in (("device", (C, G, False)), ("host", (S["C"], S["dgdp"], True))):
+        post_optimality_derivatives(s, args_[0], args_[1], as_numpy=args_[2])
         torch.cuda.synchronize()
-        tt.append((time.perf_counter() - t0) * 1e3)
-    return dict(workload="C2 KKT, p=1024 (32x32 bilinear), 64 functionals, FP64 factors",
-                ms_64_adjoints=round(float(np.median(tt)), 3), shape=list(d.shape),
-                refine_steps=s.stats()["refine_steps"])
+        tt = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            d = post_optimality_derivatives(s, args_[0], args_[1], as_numpy=args_[2])
+            torch.cuda.synchronize()
+            tt.append((time.perf_counter() - t0) * 1e3)
+        out["ms_64_adjoints" if name == "device" else "ms_64_adjoints_host_inputs"] = \
+            round(float(np.median(tt)), 3)
+        if name == "device":
+            dd = d.cpu().numpy()
+        else:
+            out["device_vs_host_max_rel_diff"] = float(np.abs(dd - d).max() / np.abs(d).max())
+    out["shape"] = list(dd.shape)
+    out["refine_steps"] = s.stats()["refine_steps"]
+    return out
 
 
 def bench_ic(torch, reps=3):

@@ -133,6 +133,14 @@ def test_c4_full_size_adjoint_vs_oracle_and_central_differences():
     err = np.abs(dfdp - ref).max() / np.abs(ref).max()
     print(f"dfdp vs oracle: max relative difference {err:.2e}")
     assert err <= 1e-9
+    # the device-resident path: inputs in HBM, (dg/dp)^T psi on the GPU
+    from amigo_amd.adjoint import DeviceJacobian
+    dd = post_optimality_derivatives(s, torch.as_tensor(C, device="cuda"), DeviceJacobian(G),
+                                     as_numpy=False)
+    assert isinstance(dd, torch.Tensor) and dd.is_cuda and tuple(dd.shape) == (64, 1024)
+    err_d = np.abs(dd.cpu().numpy() - ref).max() / np.abs(ref).max()
+    print(f"device path vs oracle: max relative difference {err_d:.2e}")
+    assert err_d <= 1e-9
     # (b) central differences: J(p) = C K^{-1} r(p), r(p) = -G p.  The
     # tolerance comes from the solves' own residuals: with psi_k = K^{-1} c_k,
     # the error of c_k^T z~ is psi_k^T (K z~ - r), at most
