@@ -811,7 +811,8 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
 constexpr int kPB = 64;        // panel width of the partitioned inverse
 constexpr int kStepRows = 64;  // forward step: rows per workgroup (one per lane)
 constexpr int kStepCols = 32;  // backward step: columns per workgroup (8 per wave)
-constexpr int kStepRhs = 8;    // right-hand sides per workgroup of a step (operands reused)
+constexpr int kStepRhs = 4;    // right-hand sides per workgroup of a step (operands reused)
+constexpr int kGatherRhs = 4;  // right-hand sides per workgroup of a large node's gather
 constexpr long long kPinvPanel = 2LL * kPB * kPB;  // doubles per panel: B^{-1}, B^{-T}
 constexpr int kPS = 2 * kPB;   // solve step width: two panels
 // doubles per solve step: both panels' B^{-1}, B^{-T}, then X and X^T (below)
@@ -985,7 +986,7 @@ __global__ void __launch_bounds__(kWave) k_panel_inv2(DevPlan P, const int* inod
 // the same step: the next step's block is unit lower triangular, so the
 // solved entry of row b1 is its y after this step's update, which tile 0
 // computes.
-template <class T>
+template <class T, int RB>
 __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
                                                    const int* wtile, int s, double* x,
                                                    int ldx, int nrhs, long long ustride,
@@ -993,8 +994,9 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   constexpr int NW = kTPB / kWave;
   constexpr int CQ = kPB / NW;  // block columns per wave (inverse products)
   constexpr int CR = kPS / NW;  // step columns per wave (row update)
-  __shared__ double xp[kPS];
-  __shared__ double part[2][NW][kWave];
+  __shared__ double ys[RB][kPS];
+  __shared__ double xp[RB][kPS];
+  __shared__ double part[2][RB][NW][kWave];
   const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
   // everything per node depends on p alone: one round of loads
   const int m = P.m[p], npiv = P.npiv[p];
@@ -1036,40 +1038,56 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   const int* vars = P.vars + io;
   const double* dv = P.dv + io;
   const double* dv2 = P.dv2 + io;
-  // kStepRhs right-hand sides per grid row, on the operands loaded once
-  const int r1 = min(nrhs, ((int)blockIdx.y + 1) * kStepRhs);
-  for (int r = blockIdx.y * kStepRhs; r < r1; r++) {
-    double* y = Y + (long long)r * ystride + io;
+  // RB right-hand sides per grid row, on the operands loaded once, all
+  // with one set of barriers: y_s of every right-hand side staged in LDS
+  // first (each sum keeps the order of the one-right-hand-side form)
+  const int r0 = blockIdx.y * RB;
+  const int nr = min(nrhs - r0, RB);
+  for (int e = tid; e < RB * kPS; e += kTPB) {
+    const int rr = e / kPS, c = e % kPS;
+    ys[rr][c] = (rr < nr && c < w) ? Y[(long long)(r0 + rr) * ystride + io + b0 + c] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rr = 0; rr < RB; rr++) {
     double sa = 0.0, sb = 0.0;
 #pragma unroll
     for (int q = 0; q < CQ; q++) {
       const int c = wv * CQ + q;
-      const double ya = c < wa ? y[b0 + c] : 0.0;
-      const double yb = c < wb ? y[b0 + kPB + c] : 0.0;
+      const double ya = ys[rr][c];
+      const double yb = ys[rr][kPB + c];
       sa += aa[q] * ya;
       sb += ax[q] * ya + ab[q] * yb;
     }
-    part[0][wv][lane] = sa;
-    part[1][wv][lane] = sb;
-    __syncthreads();
-    if (wv == 0) {
-      xp[lane] = lane < wa ? ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) +
-                                 part[0][3][lane]
-                           : 0.0;
-      xp[kPB + lane] = lane < wb ? ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) +
-                                       part[1][3][lane]
-                                 : 0.0;
-    }
-    __syncthreads();
+    part[0][rr][wv][lane] = sa;
+    part[1][rr][wv][lane] = sb;
+  }
+  __syncthreads();
+  // x_s per right-hand side: wave wv sums right-hand sides wv, wv + NW
+  for (int rr = wv; rr < RB; rr += NW) {
+    xp[rr][lane] = lane < wa ? ((part[0][rr][0][lane] + part[0][rr][1][lane]) +
+                                part[0][rr][2][lane]) + part[0][rr][3][lane]
+                             : 0.0;
+    xp[rr][kPB + lane] = lane < wb ? ((part[1][rr][0][lane] + part[1][rr][1][lane]) +
+                                      part[1][rr][2][lane]) + part[1][rr][3][lane]
+                                   : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rr = 0; rr < RB; rr++) {
     double g = 0.0;
 #pragma unroll
-    for (int q = 0; q < CR; q++) g += lrow[q] * xp[wv * CR + q];
-    part[0][wv][lane] = g;
-    __syncthreads();
+    for (int q = 0; q < CR; q++) g += lrow[q] * xp[rr][wv * CR + q];
+    part[0][rr][wv][lane] = g;
+  }
+  __syncthreads();
+  for (int rr = wv; rr < nr; rr += NW) {
+    const int r = r0 + rr;
+    double* y = Y + (long long)r * ystride + io;
     double* xr = x + (size_t)r * ldx;
-    if (wv == 0 && i < m) {
-      const double sum =
-          ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
+    if (i < m) {
+      const double sum = ((part[0][rr][0][lane] + part[0][rr][1][lane]) + part[0][rr][2][lane]) +
+                         part[0][rr][3][lane];
       if (i < npiv) {
         const double yn = y[i] - sum;
         y[i] = yn;
@@ -1077,7 +1095,7 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
           const int j = b1 - 1;
           const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
           const double dinv = 1.0 / (d11 * d22 - d21 * d21);
-          const double ta = xp[w - 1], tb = yn;
+          const double ta = xp[rr][w - 1], tb = yn;
           xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
           xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
         }
@@ -1085,26 +1103,27 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
         P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)] -= sum;
       }
     }
-    if (tile == 0 && (wv == 1 || wv == 2)) {
-      // 1x1 pivots and 2x2 pivots inside the step (also across its 64-column
-      // blocks); a pair split by the step boundary is written above (second
-      // pivot first in the step: by the step before)
-      const int jj = (wv - 1) * kPB + lane;
-      if (jj < w) {
-        const int j = b0 + jj;
-        const int ty = ptype[j];
-        if (ty == 1) {
-          xr[unpiv(vars[j])] = xp[jj] / dv[j];
-        } else if (ty == 2 && jj + 1 < w) {
-          const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
-          const double dinv = 1.0 / (d11 * d22 - d21 * d21);
-          const double ta = xp[jj], tb = xp[jj + 1];
-          xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
-          xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
-        }
+  }
+  if (tile == 0) {
+    // 1x1 pivots and 2x2 pivots inside the step (also across its 64-column
+    // blocks); a pair split by the step boundary is written above (second
+    // pivot first in the step: by the step before)
+    for (int e = tid; e < nr * kPS; e += kTPB) {
+      const int rr = e / kPS, jj = e % kPS;
+      if (jj >= w) continue;
+      double* xr = x + (size_t)(r0 + rr) * ldx;
+      const int j = b0 + jj;
+      const int ty = ptype[j];
+      if (ty == 1) {
+        xr[unpiv(vars[j])] = xp[rr][jj] / dv[j];
+      } else if (ty == 2 && jj + 1 < w) {
+        const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
+        const double dinv = 1.0 / (d11 * d22 - d21 * d21);
+        const double ta = xp[rr][jj], tb = xp[rr][jj + 1];
+        xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
+        xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
       }
     }
-    __syncthreads();
   }
 }
 
@@ -1115,7 +1134,7 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
 //   y[c] -= L[b0:b1, c]^T x  for its kStepCols columns c < b0
 // (one wave per column, lanes over the step's rows, two per lane).  Tile 0
 // writes x.
-template <class T>
+template <class T, int RB>
 __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
                                                    const int* wtile, int s, double* x,
                                                    int ldx, int nrhs, double* Y,
@@ -1123,8 +1142,9 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
   constexpr int NW = kTPB / kWave;
   constexpr int CQ = kPB / NW;         // block rows per wave (inverse products)
   constexpr int CPW = kStepCols / NW;  // tile columns per wave
-  __shared__ double xp[kPS];
-  __shared__ double part[2][NW][kWave];
+  __shared__ double ys[RB][kPS];
+  __shared__ double xp[RB][kPS];
+  __shared__ double part[2][RB][NW][kWave];
   const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
   // everything per node depends on p alone: one round of loads
   const int m = P.m[p], npiv = P.npiv[p];
@@ -1163,45 +1183,64 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
     tb[k] = i < wb ? invbT[lane + i * kPB] : 0.0;
   }
   const int* vars = P.vars + io;
-  // kStepRhs right-hand sides per grid row, on the operands loaded once
-  const int r1 = min(nrhs, ((int)blockIdx.y + 1) * kStepRhs);
-  for (int r = blockIdx.y * kStepRhs; r < r1; r++) {
-    double* y = Y + (long long)r * ystride + io;
+  // RB right-hand sides per grid row, on the operands loaded once, with one
+  // set of barriers (each sum in the order of the one-right-hand-side form)
+  const int r0 = blockIdx.y * RB;
+  const int nr = min(nrhs - r0, RB);
+  for (int e = tid; e < RB * kPS; e += kTPB) {
+    const int rr = e / kPS, c = e % kPS;
+    ys[rr][c] = (rr < nr && c < w) ? Y[(long long)(r0 + rr) * ystride + io + b0 + c] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rr = 0; rr < RB; rr++) {
     double sa = 0.0, sb = 0.0;
 #pragma unroll
     for (int k = 0; k < CQ; k++) {
       const int i = wv * CQ + k;
-      const double ya = i < wa ? y[b0 + i] : 0.0;
-      const double yb = i < wb ? y[b0 + kPB + i] : 0.0;
+      const double ya = ys[rr][i];
+      const double yb = ys[rr][kPB + i];
       sa += ta[k] * ya + tx[k] * yb;
       sb += tb[k] * yb;
     }
-    part[0][wv][lane] = sa;
-    part[1][wv][lane] = sb;
-    __syncthreads();
-    if (wv == 0) {
-      xp[lane] = lane < wa ? ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) +
-                                 part[0][3][lane]
-                           : 0.0;
-      xp[kPB + lane] = lane < wb ? ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) +
-                                       part[1][3][lane]
-                                 : 0.0;
-    }
-    __syncthreads();
-    const double xl0 = xp[lane], xl1 = xp[kPB + lane];
+    part[0][rr][wv][lane] = sa;
+    part[1][rr][wv][lane] = sb;
+  }
+  __syncthreads();
+  for (int rr = wv; rr < RB; rr += NW) {
+    xp[rr][lane] = lane < wa ? ((part[0][rr][0][lane] + part[0][rr][1][lane]) +
+                                part[0][rr][2][lane]) + part[0][rr][3][lane]
+                             : 0.0;
+    xp[rr][kPB + lane] = lane < wb ? ((part[1][rr][0][lane] + part[1][rr][1][lane]) +
+                                      part[1][rr][2][lane]) + part[1][rr][3][lane]
+                                   : 0.0;
+  }
+  __syncthreads();
+  // the wave's CPW columns for every right-hand side: full sums in every
+  // lane, then lane rr * CPW + k updates column k of right-hand side rr (one
+  // read-modify-write per lane, all in flight together)
+  static_assert(RB * CPW <= kWave, "one lane per (right-hand side, column)");
+  double gsel = 0.0;
+#pragma unroll
+  for (int rr = 0; rr < RB; rr++) {
+    const double xl0 = xp[rr][lane], xl1 = xp[rr][kPB + lane];
 #pragma unroll
     for (int k = 0; k < CPW; k++) {
       double g = lcol[k][0] * xl0 + lcol[k][1] * xl1;
       for (int off = kWave / 2; off > 0; off >>= 1) g += __shfl_xor(g, off);
-      const int c = c0 + wv * CPW + k;
-      if (lane == 0 && c < b0) y[c] -= g;
+      if (lane == rr * CPW + k) gsel = g;
     }
-    if (tile == 0 && (wv == 1 || wv == 2)) {
-      const int jj = (wv - 1) * kPB + lane;
-      if (jj < w) x[(size_t)r * ldx + unpiv(vars[b0 + jj])] = xp[jj];
-    }
-    __syncthreads();
   }
+  {
+    const int rr = lane / CPW, k = lane % CPW;
+    const int c = c0 + wv * CPW + k;
+    if (rr < nr && c < b0) Y[(long long)(r0 + rr) * ystride + io + c] -= gsel;
+  }
+  if (tile == 0)
+    for (int e = tid; e < nr * kPS; e += kTPB) {
+      const int rr = e / kPS, jj = e % kPS;
+      if (jj < w) x[(size_t)(r0 + rr) * ldx + unpiv(vars[b0 + jj])] = xp[rr][jj];
+    }
 }
 
 // Pivot columns of a BIG node, backward: Y[j] = x[pivot j] - sum_{i >= npiv}
@@ -1209,12 +1248,12 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
 // (4 per wave, lanes over the rows, coalesced), with the node's x rows
 // staged in LDS.
 constexpr int kGemvCols = 16;
-template <class T>
+template <class T, int RB>
 __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
                                                    const int* wtile, const double* x,
                                                    int ldx, int nrhs, double* Y,
-                                                   long long ystride) {
-  extern __shared__ double xs[];  // x of the rows below the pivots
+                                                   long long ystride, int ldxs) {
+  extern __shared__ double xs[];  // x of the rows below the pivots, RB right-hand sides
   const int p = wnode[blockIdx.x];
   const int m = P.m[p], npiv = P.npiv[p];
   const int j0 = kGemvCols * wtile[blockIdx.x];
@@ -1225,64 +1264,157 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const T* L = front_ptr<T>(P, p);
   const int nr = m - npiv;
-  {
-    const int r = blockIdx.y;  // one right-hand side per grid row
-    const double* xr = x + (size_t)r * ldx;
-    // both index loads issued (clamped into their arrays; a node without
-    // contribution rows reads vars twice), then the gather
-    const int cs = P.cs[p];
-    const int* crow = cs > 0 ? P.contrib_rows + cp : vars;
-    for (int q = tid; q < nr; q += kTPB) {
-      const int i = npiv + q;
-      const int iv = vars[min(i, fs - 1)];
-      const int ic = crow[min(max(i - fs, 0), max(cs - 1, 0))];
-      xs[q] = xr[i < fs ? iv : ic];
+  const int r0 = blockIdx.y * RB;
+  const int nb = min(nrhs - r0, RB);
+  // both index loads issued (clamped into their arrays; a node without
+  // contribution rows reads vars twice), then the gather
+  const int cs = P.cs[p];
+  const int* crow = cs > 0 ? P.contrib_rows + cp : vars;
+  for (int q = tid; q < nr; q += kTPB) {
+    const int i = npiv + q;
+    const int iv = vars[min(i, fs - 1)];
+    const int ic = crow[min(max(i - fs, 0), max(cs - 1, 0))];
+    const int v = i < fs ? iv : ic;
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++)
+      if (rr < nb) xs[rr * ldxs + q] = x[(size_t)(r0 + rr) * ldx + v];
+  }
+  __syncthreads();
+  // the wave's four columns together, so that their loads are in flight
+  // at once (columns past npiv read column npiv - 1 and are not written);
+  // each column accumulates over q = lane, lane + 64, ... in order, for
+  // every right-hand side on the same L loads
+  constexpr int CW = kGemvCols / 4;
+  const int jb = j0 + 4 * w;
+  const T* Lc[CW];
+#pragma unroll
+  for (int jj = 0; jj < CW; jj++) Lc[jj] = L + (size_t)min(jb + jj, npiv - 1) * m + npiv;
+  double sc[CW][RB];
+#pragma unroll
+  for (int jj = 0; jj < CW; jj++)
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++) sc[jj][rr] = 0.0;
+  int q = lane;
+  for (; q + kWave < nr; q += 2 * kWave) {
+    T a[CW][2];
+#pragma unroll
+    for (int jj = 0; jj < CW; jj++) {
+      a[jj][0] = Lc[jj][q];
+      a[jj][1] = Lc[jj][q + kWave];
     }
-    __syncthreads();
-    // the wave's four columns together, so that their loads are in flight
-    // at once (columns past npiv read column npiv - 1 and are not written);
-    // each column accumulates over q = lane, lane + 64, ... in order
-    constexpr int CW = kGemvCols / 4;
-    const int jb = j0 + 4 * w;
-    const T* Lc[CW];
 #pragma unroll
-    for (int jj = 0; jj < CW; jj++) Lc[jj] = L + (size_t)min(jb + jj, npiv - 1) * m + npiv;
-    double sc[CW];
-#pragma unroll
-    for (int jj = 0; jj < CW; jj++) sc[jj] = 0.0;
-    int q = lane;
-    for (; q + kWave < nr; q += 2 * kWave) {
-      T a[CW][2];
+    for (int rr = 0; rr < RB; rr++) {
+      const double x0 = xs[rr * ldxs + q], x1 = xs[rr * ldxs + q + kWave];
 #pragma unroll
       for (int jj = 0; jj < CW; jj++) {
-        a[jj][0] = Lc[jj][q];
-        a[jj][1] = Lc[jj][q + kWave];
-      }
-      const double x0 = xs[q], x1 = xs[q + kWave];
-#pragma unroll
-      for (int jj = 0; jj < CW; jj++) {
-        sc[jj] = __builtin_fma((double)a[jj][0], x0, sc[jj]);
-        sc[jj] = __builtin_fma((double)a[jj][1], x1, sc[jj]);
+        sc[jj][rr] = __builtin_fma((double)a[jj][0], x0, sc[jj][rr]);
+        sc[jj][rr] = __builtin_fma((double)a[jj][1], x1, sc[jj][rr]);
       }
     }
-    if (q < nr) {
-      const double x0 = xs[q];
+  }
+  if (q < nr) {
 #pragma unroll
-      for (int jj = 0; jj < CW; jj++) sc[jj] = __builtin_fma((double)Lc[jj][q], x0, sc[jj]);
+    for (int rr = 0; rr < RB; rr++) {
+      const double x0 = xs[rr * ldxs + q];
+#pragma unroll
+      for (int jj = 0; jj < CW; jj++)
+        sc[jj][rr] = __builtin_fma((double)Lc[jj][q], x0, sc[jj][rr]);
     }
+  }
+#pragma unroll
+  for (int jj = 0; jj < CW; jj++)
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++)
+      for (int off = kWave / 2; off > 0; off >>= 1) sc[jj][rr] += __shfl_xor(sc[jj][rr], off);
+  // every lane holds the sums; lane rr * CW + jj writes column jj of
+  // right-hand side rr
+  static_assert(RB * CW <= kWave, "one lane per (right-hand side, column)");
+  double v = 0.0;
+#pragma unroll
+  for (int rr = 0; rr < RB; rr++)
 #pragma unroll
     for (int jj = 0; jj < CW; jj++)
-      for (int off = kWave / 2; off > 0; off >>= 1) sc[jj] += __shfl_xor(sc[jj], off);
-    // every lane holds the sums; lane jj writes column jj
-    if (lane < CW) {
-      double v = sc[0];
-#pragma unroll
-      for (int jj = 1; jj < CW; jj++)
-        if (lane == jj) v = sc[jj];
-      const int j = jb + lane;
-      if (j < npiv) Y[(long long)r * ystride + P.ioff[p] + j] = xr[unpiv(vars[j])] - v;
+      if (lane == rr * CW + jj) v = sc[jj][rr];
+  {
+    const int rr = lane / CW, j = jb + lane % CW;
+    if (rr < nb && j < npiv) {
+      const double* xr = x + (size_t)(r0 + rr) * ldx;
+      Y[(long long)(r0 + rr) * ystride + P.ioff[p] + j] = xr[unpiv(vars[j])] - v;
     }
-    __syncthreads();
+  }
+}
+
+// Forward gather of a BIG node for RB right-hand sides per grid row: the
+// pivot rows of the front vector to Y, the rows below to U (the panel steps
+// solve them).  Each row's child lookups are done once for all the
+// workgroup's right-hand sides; each entry sums its children in child order
+// (the order of k_forward's gather).
+template <class T, int RB>
+__global__ void __launch_bounds__(1024) k_gather_big(DevPlan P, const int* nodes, double* x,
+                                                     int ldx, int nrhs, long long ustride,
+                                                     double* Y, long long ystride) {
+  constexpr int NTH = 1024;
+  const int p = nodes[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int m = P.m[p], npiv = P.npiv[p];
+  const int ns = P.ns[p], din = P.din[p];
+  const int fs = ns + din;
+  const long long io = P.ioff[p];
+  const int* pre = P.pre + io;
+  const int c0 = P.chptr[p], c1 = P.chptr[p + 1];
+  const int vp = P.vptr[p];
+  __shared__ ChildView<T> s_cv[kChildCache];
+  __shared__ long long s_uo[kChildCache];
+  const int nch = c1 - c0;
+  for (int q = tid; q < min(nch, kChildCache); q += NTH) {
+    s_cv[q] = load_child<T>(P, P.ch[c0 + q]);
+    s_uo[q] = P.uoff[P.ch[c0 + q]];
+  }
+  __syncthreads();
+  const int r0 = blockIdx.y * RB;
+  const int nb = min(nrhs - r0, RB);
+  double* U = P.U + P.uoff[p];
+  for (int i = tid; i < m; i += NTH) {
+    const int pi = i < fs ? pre[i] : i;
+    const int var = P.node_to_var[vp + min(pi, ns - 1)];
+    double v[RB];
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++) {
+      const double xv = x[(size_t)(r0 + min(rr, nb - 1)) * ldx + var];
+      v[rr] = pi < ns ? xv : 0.0;
+    }
+    for (int q0 = 0; q0 < nch; q0 += 4) {
+      int rc[4];
+      ChildView<T> cvs[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int qq = min(q0 + u, nch - 1);
+        cvs[u] = qq < kChildCache ? s_cv[qq] : load_child<T>(P, P.ch[c0 + qq]);
+        const int rq = child_row_nb(P, cvs[u], pi, ns, din);
+        rc[u] = q0 + u < nch ? rq : -1;
+      }
+#pragma unroll
+      for (int rr = 0; rr < RB; rr++) {
+        double uv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int qq = min(q0 + u, nch - 1);
+          const long long uo = qq < kChildCache ? s_uo[qq] : P.uoff[P.ch[c0 + qq]];
+          uv[u] = P.U[uo + (long long)(r0 + min(rr, nb - 1)) * ustride +
+                      max(rc[u] - cvs[u].npiv, 0)];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (rc[u] >= 0) v[rr] += uv[u];
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++) {
+      if (rr >= nb) break;
+      const long long r = r0 + rr;
+      if (i < npiv) Y[r * ystride + io + i] = v[rr];
+      else U[r * ustride + (i - npiv)] = v[rr];
+    }
   }
 }
 
@@ -2476,16 +2608,26 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
-      hipLaunchKernelGGL((k_forward<T, kGatherThreads, true>), dim3(g1 - g0, nrhs),
-                         dim3(kGatherThreads), sizeof(double) * (size_t)lds_t, st, P,
-                         H.d_large_nodes.p + g0, dx, ldx, nrhs, us, chol, Y, ys);
+      if (nrhs == 1)
+        hipLaunchKernelGGL((k_forward<T, kGatherThreads, true>), dim3(g1 - g0, nrhs),
+                           dim3(kGatherThreads), sizeof(double) * (size_t)lds_t, st, P,
+                           H.d_large_nodes.p + g0, dx, ldx, nrhs, us, chol, Y, ys);
+      else
+        hipLaunchKernelGGL((k_gather_big<T, kGatherRhs>),
+                           dim3(g1 - g0, (nrhs + kGatherRhs - 1) / kGatherRhs), dim3(kGatherThreads),
+                           0, st, P, H.d_large_nodes.p + g0, dx, ldx, nrhs, us, Y, ys);
       const int nst = std::min(H.fs_lvl[l + 1] - H.fs_lvl[l], H.lvl_steps[l]);
       for (int t = 0; t < nst; t++) {
         const int slot = H.fs_lvl[l] + t;
         const int f0 = H.fs_ptr[slot], f1 = H.fs_ptr[slot + 1];
-        hipLaunchKernelGGL(k_fwd_step<T>, dim3(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs),
-                           dim3(kTPB), 0, st, P,
-                           H.d_fs_node.p + f0, H.d_fs_tile.p + f0, t, dx, ldx, nrhs, us, Y, ys);
+        if (nrhs == 1)
+          hipLaunchKernelGGL((k_fwd_step<T, 1>), dim3(f1 - f0, 1), dim3(kTPB), 0, st, P,
+                             H.d_fs_node.p + f0, H.d_fs_tile.p + f0, t, dx, ldx, nrhs, us, Y, ys);
+        else
+          hipLaunchKernelGGL((k_fwd_step<T, kStepRhs>),
+                             dim3(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs), dim3(kTPB), 0, st,
+                             P, H.d_fs_node.p + f0, H.d_fs_tile.p + f0, t, dx, ldx, nrhs, us, Y,
+                             ys);
       }
     }
   }
@@ -2494,15 +2636,22 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
       const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
-      hipLaunchKernelGGL(k_bwd_gemv<T>, dim3(b1 - b0, nrhs), dim3(kTPB), lds_bg, st, P,
-                         H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys);
+      // one right-hand side per workgroup (batching them here serialises a
+      // workgroup's dot products and was measured slower)
+      hipLaunchKernelGGL((k_bwd_gemv<T, 1>), dim3(b1 - b0, nrhs), dim3(kTPB), lds_bg, st, P,
+                         H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys,
+                         H.max_cap);
       const int nst = std::min(H.bs_lvl[l + 1] - H.bs_lvl[l], H.lvl_steps[l]);
       for (int t = 0; t < nst; t++) {
         const int slot = H.bs_lvl[l] + t;
         const int f0 = H.bs_ptr[slot], f1 = H.bs_ptr[slot + 1];
-        hipLaunchKernelGGL(k_bwd_step<T>, dim3(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs),
-                           dim3(kTPB), 0, st, P,
-                           H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
+        if (nrhs == 1)
+          hipLaunchKernelGGL((k_bwd_step<T, 1>), dim3(f1 - f0, 1), dim3(kTPB), 0, st, P,
+                             H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
+        else
+          hipLaunchKernelGGL((k_bwd_step<T, kStepRhs>),
+                             dim3(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs), dim3(kTPB), 0, st,
+                             P, H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
       }
     }
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
