@@ -1602,6 +1602,8 @@ struct Handle {
   // LDS fronts take the right-looking kernels of kkt_front.hip (every mode
   // but Cholesky, which keeps k_front_lds)
   bool rl_fronts = true;
+  // large root fronts keep Bunch-Kaufman pivoting throughout (mode 1)
+  bool root_bk = false;
   int n_lds_ev = 0;
   std::vector<hipEvent_t> lds_ev;
   std::vector<int> tk_grp_ptr, tk_grp_nodes, tk_lds_ptr, tk_lds_nodes, tk_small_ptr,
@@ -1669,6 +1671,10 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     H.cap[k] = H.ns[k] + H.cs[k] + slack[k];
     bool root = S.node_parent[k] < 0;
     H.mode[k] = H.opt.solver_type == KKT_LDL_CHOLESKY ? 2 : (root ? 1 : 0);
+    // a root front too large for LDS: threshold pivoting with delays first
+    // (the tests the speculative panel accepts), Bunch-Kaufman on what it
+    // delays (mode 3); KKT_LDL_ROOT_BK=1 keeps Bunch-Kaufman throughout
+    if (H.mode[k] == 1 && !H.root_bk && H.cap[k] > H.lds_front) H.mode[k] = 3;
   }
   // levels (leaves = 0)
   H.level.assign(N, 0);
@@ -1714,7 +1720,10 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   // launches suffice, and the update tiles of a front's panel t may start
   // at the smaller bound t*(kNBLarge-2).
   auto is_large = [&](int k) { return H.cap[k] > H.lds_front && H.mode[k] != 2; };
-  auto panels_of = [&](int k) { return (H.cap[k] - H.cs[k] + kNBLarge - 2) / (kNBLarge - 1); };
+  // (a mode-3 root may end its threshold phase with a short panel: one more)
+  auto panels_of = [&](int k) {
+    return (H.cap[k] - H.cs[k] + kNBLarge - 2) / (kNBLarge - 1) + (H.mode[k] == 3 ? 1 : 0);
+  };
   H.small_ptr.assign(nlev + 1, 0);
   H.large_ptr.assign(nlev + 1, 0);
   H.lds_ptr.assign(nlev + 1, 0);
@@ -1826,8 +1835,11 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
       for (int k : act[t]) {
         H.tk_act_nodes.push_back(k);
         H.tk_act_cap[t] = std::max(H.tk_act_cap[t], H.cap[k]);
-        H.tk_act_root[t] |= H.mode[k] == 1;
-        const int base = (t - start[k]) * (kNBLarge - 2);
+        H.tk_act_root[t] |= H.mode[k] == 1 || H.mode[k] == 3;
+        // first column of the tile grid: a lower bound of the panel's start
+        // (one short panel allowed for a mode-3 root)
+        const int tp = t - start[k] - (H.mode[k] == 3 ? 1 : 0);
+        const int base = std::max(tp, 0) * (kNBLarge - 2);
         const int nt = (H.cap[k] - base + 63) / 64;
         for (int tt = 0; tt < nt * (nt + 1) / 2; tt++) {
           H.upd_node.push_back(k);
@@ -2844,6 +2856,7 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   H.rl_fronts = H.opt.solver_type != KKT_LDL_CHOLESKY;
   if (const char* e = std::getenv("KKT_LDL_RL_FRONTS")) H.rl_fronts &= std::atoi(e) != 0;
   H.lds_front = std::min(H.lds_front, rl_max_front());
+  if (const char* e = std::getenv("KKT_LDL_ROOT_BK")) H.root_bk = std::atoi(e) != 0;
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));
   H.stats.n = n;

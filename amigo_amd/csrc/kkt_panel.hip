@@ -31,9 +31,16 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
   __syncthreads();
   int k = s_st.k, nc = s_st.nc, kw = 0;
   const int kstart = k;
+  // a root front in mode 3 turns to Bunch-Kaufman once its threshold phase
+  // has left only delayed columns (the phase flag is kept in the state)
+  int bkphase = s_st.pad;
+  if (P.mode[p] == 3 && !bkphase && k >= nc && nc < P.m[p]) {
+    nc = P.m[p];
+    bkphase = 1;
+  }
   if (k < nc) {
     const int m = P.m[p];
-    const int mode = P.mode[p];
+    const int mode = P.mode[p] == 3 ? bkphase : P.mode[p];
     const int nb = max(2, min(kNBLarge, m));
     const long long io = P.ioff[p];
     int* vars = s_iv;
@@ -68,6 +75,7 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
     st.npos = sh.npos;
     st.nneg = sh.nneg;
     st.n2 = sh.n2;
+    st.pad = bkphase;
     P.ps[p] = st;
   }
 }
@@ -109,7 +117,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
   const int stat = P.stat[p];
   const PanelState st0 = P.ps[p];
   const int m = P.m[p];
-  const int mode = P.mode[p];
+  const int mode0 = P.mode[p];
   const long long io = P.ioff[p];
   T* F = front_ptr<T>(P, p);
   double* Wg = P.W + P.woff[p];
@@ -123,6 +131,15 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
     sh.n2 = st0.n2;
   }
   int k = st0.k, nc = st0.nc, kw = 0;
+  // mode 3 (a large root front): threshold tests with delays while
+  // candidates remain, then Bunch-Kaufman on the delayed columns; the switch
+  // is made at the start of a panel and kept in the state
+  int bkphase = st0.pad;
+  if (mode0 == 3 && !bkphase && k >= nc && nc < m) {
+    nc = m;
+    bkphase = 1;
+  }
+  const int mode = mode0 == 3 ? bkphase : mode0;
   const int kstart = k;
   if (k < nc) {
     const int nb = max(2, min(NB, m));
@@ -910,6 +927,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
     st.npos = sh.npos;
     st.nneg = sh.nneg;
     st.n2 = sh.n2;
+    st.pad = bkphase;
     P.ps[p] = st;
   }
 }

@@ -89,3 +89,41 @@ def test_right_looking_matches_blocked_lds_kernel(name, kw, order, prec):
         assert np.array_equal(n1[k], n0[k]), k
     assert np.abs(x1 - x0).max() <= 1e-8 * np.abs(x0).max()
     assert _relres(P.matrix(), x1, P.rhs) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
+def test_large_root_threshold_then_bunch_kaufman(prec):
+    """A root front too large for LDS is factored with the threshold tests
+    first and Bunch-Kaufman on the columns they delay (mode 3); with
+    KKT_LDL_ROOT_BK=1 it keeps Bunch-Kaufman throughout.  Both give the
+    oracle's inertia (the reference's root is LAPACK's Bunch-Kaufman; the
+    inertia does not depend on the pivot sequence), the same pivot outcome
+    below the root, all of the root's columns pivoted, and refined
+    residuals <= 1e-10 on the full C2 KKT under nested dissection."""
+    print("root mode", prec, flush=True)
+    P = W.poisson2d()
+    res = {}
+    for flag in ("0", "1"):
+        os.environ["KKT_LDL_ROOT_BK"] = flag
+        try:
+            s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
+                          order=OrderingType.NESTED_DISSECTION, precision=prec, max_refine=30)
+        finally:
+            del os.environ["KKT_LDL_ROOT_BK"]
+        assert s.factor(P.diag) == 0
+        x = P.rhs.copy()
+        s.solve(x, refine=True)
+        assert _relres(P.matrix(), x, P.rhs) <= 1e-10
+        res[flag] = (s.get_inertia(), s.node_info(), x, s.get_perm()[0])
+    o = OracleLDL(P.n, P.rowp, P.cols, order=GIVEN, perm=res["0"][3])
+    o.factor(P.vals, P.diag)
+    b = o.node_info()
+    for flag in ("0", "1"):
+        inertia, info, _, _ = res[flag]
+        assert inertia == o.inertia()
+        root = b["root"] != 0
+        assert np.array_equal(info["npiv"], b["npiv"])
+        assert info["ndel"][root].sum() == 0
+        for k in ("ndel", "n2x2"):
+            assert np.array_equal(info[k][~root], b[k][~root]), k
+    assert np.abs(res["0"][2] - res["1"][2]).max() <= 1e-8 * np.abs(res["1"][2]).max()
