@@ -212,10 +212,12 @@ def bench_multi_rhs(r, P, torch, nrhs=64, reps=3):
     return out
 
 
-FACTOR_KERNELS = ("k_prepare", "k_assemble", "k_front_lds", "k_factor", "k_panel_rr", "k_panel",
-                  "k_update", "k_panel_inv", "k_panel_inv2", "k_pivot_check")
-SOLVE_KERNELS = ("k_forward", "k_backward", "k_fwd_step", "k_bwd_step", "k_bwd_gemv",
-                 "k_residual", "k_colnorm2_part", "k_colnorm2_final", "k_axpy_cols")
+FACTOR_KERNELS = ("k_prepare", "k_assemble", "k_front_lds", "k_front_rl", "k_front_chain",
+                  "k_factor", "k_panel_rr", "k_panel", "k_update", "k_panel_inv", "k_panel_inv2",
+                  "k_pivot_check")
+SOLVE_KERNELS = ("k_forward", "k_backward", "k_gather_big", "k_fwd_step", "k_bwd_step",
+                 "k_bwd_gemv", "k_residual", "k_colnorm2_part", "k_colnorm2_final",
+                 "k_axpy_cols")
 
 
 def _kernel_base(name):
