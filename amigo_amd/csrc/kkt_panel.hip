@@ -131,6 +131,7 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
     sh.n2 = st0.n2;
   }
   int k = st0.k, nc = st0.nc, kw = 0;
+  int kw_spec = 0;  // columns whose L the speculation stored
   // mode 3 (a large root front): threshold tests with delays while
   // candidates remain, then Bunch-Kaufman on the delayed columns; the switch
   // is made at the start of a panel and kept in the state
@@ -424,6 +425,21 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
       const int nok = fm ? min(nsp, (int)__builtin_ctz(fm)) : nsp;
       kw = nok < nsp ? sh.sp_c[nok] : sh.sp_end;
       k = kstart + kw;
+      kw_spec = kw;
+      // the accepted columns' L entries are final: stored now (ordered
+      // before the sequential loop by its full barrier, so that its HBM
+      // swaps move them as they move the rest of the rows), not at the end
+      for (int q = 0; q < nok; q++) {
+        const int c = sh.sp_c[q], t = sh.sp_t[q], pos = kstart + c;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          const int r = rowof(j);
+          if (r > pos + t - 1 && r < m) {
+            F[r + (size_t)pos * ld] = lsx(c, j);
+            if (t == 2) F[r + (size_t)(pos + 1) * ld] = lsx(c + 1, j);
+          }
+        }
+      }
       // the accepted steps' D entries, pivot types and inertia, one step per
       // thread (the counts are sums)
       if (tid < nok) {
@@ -895,8 +911,8 @@ __global__ void __launch_bounds__(NT) k_panel_rr(DevPlan P, const int* nodes) {
     }
     lds_sync();
     // L of the panel below each pivot block (the D entries were written at
-    // the pivot)
-    for (int c = 0; c < kw; c++) {
+    // the pivot), for the columns of the sequential loop
+    for (int c = kw_spec; c < kw; c++) {
       const int pos = kstart + c;
       const int last = (sh.cT[c] == 2) ? pos + 1 : pos;
 #pragma unroll
