@@ -127,3 +127,22 @@ def test_large_root_threshold_then_bunch_kaufman(prec):
         for k in ("ndel", "n2x2"):
             assert np.array_equal(info[k][~root], b[k][~root]), k
     assert np.abs(res["0"][2] - res["1"][2]).max() <= 1e-8 * np.abs(res["1"][2]).max()
+
+
+def test_chain_overflow_replans():
+    """Minimal front slack on the AMD cart-pole chain: fronts overflow inside
+    the single k_front_chain launch (each failed front marks its ancestors
+    failed within the same launch), the host enlarges the slack and
+    refactors; the result matches the oracle node by node."""
+    P = W.cartpole(N=3000)
+    s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals), order=OrderingType.AMD,
+                  delay_growth=1.0)
+    assert s.factor(P.diag) == 0
+    assert s.stats()["replans"] > 0
+    o = OracleLDL(P.n, P.rowp, P.cols, order=AMD)
+    o.factor(P.vals, P.diag)
+    assert s.get_inertia() == o.inertia()
+    _node_parity(s, o)
+    x = P.rhs.copy()
+    s.solve(x, refine=True)
+    assert _relres(P.matrix(), x, P.rhs) <= 1e-10
