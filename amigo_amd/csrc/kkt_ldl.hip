@@ -260,6 +260,191 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
       F[(size_t)j * m + j] = (T)((double)F[(size_t)j * m + j] + P.diag[P.node_to_var[vp + j]]);
 }
 
+// Assembly with the workgroup's kAsmCols columns staged in LDS (front
+// element type, leading dimension ldc >= m): the gather writes LDS only, so
+// no global store sits between one row's loads and the next row's (vector
+// memory operations complete in order), the original entries and Sigma are
+// added there with k_assemble's roundings, and the columns go to the front
+// in one pass.  The node is prepared here (the children's delay offsets by
+// a prefix sum over one wave; the workgroup of chunk 0 writes the node
+// state).  For nodes with at most kChildCache children and fronts of at
+// most kAsmLdsRows rows; bitwise equal to k_assemble.
+constexpr int kAsmLdsRows = 2048;
+constexpr int kAsmLdsCols = 4;  // columns per workgroup of k_assemble_lds
+template <class T>
+__global__ void __launch_bounds__(kTPB) k_assemble_lds(DevPlan P, const int* wnode,
+                                                       const int* wchunk, int ldc) {
+  extern __shared__ unsigned char s_raw[];
+  T* colv = (T*)s_raw;
+  const int p = wnode[blockIdx.x];
+  const int chunk = wchunk[blockIdx.x];
+  const int tid = threadIdx.x, wv = tid / kWave, ln = tid % kWave;
+  const int c0 = P.chptr[p], nch = P.chptr[p + 1] - c0;
+  const int ns = P.ns[p], cs = P.cs[p];
+  const int vp = P.vptr[p];
+  const int j0 = chunk * kAsmLdsCols;
+  __shared__ ChildView<T> s_cv[kChildCache];
+  __shared__ int s_list[kAsmLdsCols][kChildCache];
+  __shared__ int s_rj[kAsmLdsCols][kChildCache];
+  __shared__ int s_nl[kAsmLdsCols];
+  __shared__ int s_fail, s_din;
+  // the original entries of the own columns in [j0, j0 + kAsmLdsCols) and
+  // their Sigma terms: loaded now, in flight during the preparation
+  int e0 = 0, e1 = 0;
+  if (j0 < ns) {
+    e0 = P.am_ptr[vp + j0];
+    e1 = P.am_ptr[vp + min(j0 + kAsmLdsCols, ns)];
+  }
+  int oe_u = -1, oe_j = 0;
+  double oe_v = 0.0;
+  if (e0 + tid < e1) {
+    const int e = e0 + tid;
+    oe_u = P.am_row[e];
+    oe_j = P.am_col[e];
+    oe_v = P.vals[P.am_src[e]];
+  }
+  double dg = 0.0;
+  if (P.diag && tid < kAsmLdsCols && j0 + tid < ns) dg = P.diag[P.node_to_var[vp + j0 + tid]];
+  if (wv == 0) {
+    // children's delay counts in child order: offsets by an exclusive prefix
+    // sum over the wave (a failed child adds nothing and fails the node)
+    int c = 0, st = 0, nd = 0;
+    if (ln < nch) {
+      c = P.ch[c0 + ln];
+      st = P.stat[c];
+      nd = P.ndel[c];
+    }
+    const int nda = st ? 0 : nd;
+    int incl = nda;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(incl, off);
+      if (ln >= off) incl += y;
+    }
+    const int doff = incl - nda;
+    const int din = __shfl(incl, kWave - 1);
+    const bool cfail = __ballot(ln < nch && st != 0) != 0ull;
+    if (ln < nch) {
+      ChildView<T> v;
+      v.F = front_ptr<T>(P, c);
+      v.m = P.m[c];
+      v.fs = P.ns[c] + P.din[c];
+      v.npiv = P.npiv[c];
+      v.ndel = nd;
+      v.doff = doff;
+      v.inv = P.invoff[c];
+      s_cv[ln] = v;
+      if (chunk == 0 && !st) P.doff[c] = doff;
+    }
+    if (ln == 0) {
+      const int m = ns + din + cs;
+      const int fail = cfail || m > P.cap[p];
+      if (chunk == 0) {
+        P.din[p] = din;
+        P.m[p] = m;
+        P.npiv[p] = 0;
+        P.ndel[p] = 0;
+        P.stat[p] = fail;
+        if (fail) P.need[p] = cfail ? -1 : m;
+        PanelState ps;
+        ps.k = 0;
+        ps.nc = P.mode[p] == 1 ? m : ns + din;
+        ps.kstart = 0;
+        ps.kw = 0;
+        ps.npos = ps.nneg = ps.n2 = ps.pad = 0;
+        P.ps[p] = ps;
+      }
+      s_fail = fail;
+      s_din = din;
+    }
+  }
+  __syncthreads();
+  if (s_fail) return;
+  const int din = s_din;
+  const int m = ns + din + cs;
+  if (j0 >= m) return;
+  const int j1 = min(m, j0 + kAsmLdsCols);
+  if (chunk == 0) {
+    int* vars = P.vars + P.ioff[p];
+    int* pre = P.pre + P.ioff[p];
+    const int cp = P.cptr[p];
+    for (int i = tid; i < ns; i += kTPB) vars[i] = P.node_to_var[vp + i];
+    for (int i = tid; i < cs; i += kTPB) vars[ns + din + i] = P.contrib_rows[cp + i];
+    for (int i = tid; i < m; i += kTPB) pre[i] = i;
+    for (int q = 0; q < nch; q++) {
+      const ChildView<T>& cv = s_cv[q];
+      const int* cvv = P.vars + P.ioff[P.ch[c0 + q]] + cv.npiv;
+      for (int t = tid; t < cv.ndel; t += kTPB) vars[ns + cv.doff + t] = cvv[t];
+    }
+  }
+  // the children that own each column, compacted in child order: wave w for
+  // column j0 + w
+  for (int cc = wv; cc < kAsmLdsCols; cc += kTPB / kWave) {
+    const int j = j0 + cc;
+    const int rj = (ln < nch && j < j1) ? child_row(P, s_cv[ln], j, ns, din) : -1;
+    const unsigned long long mask = __ballot(rj >= 0);
+    const int pos = __popcll(mask & ((1ull << ln) - 1ull));
+    if (rj >= 0) {
+      s_list[cc][pos] = ln;
+      s_rj[cc][pos] = rj;
+    }
+    if (ln == 0) s_nl[cc] = __popcll(mask);
+  }
+  __syncthreads();
+  for (int jj = 0; jj < j1 - j0; jj++) {
+    const int j = j0 + jj;
+    const int nl = s_nl[jj];
+    T* cj = colv + (size_t)jj * ldc;
+    for (int i = j + tid; i < m; i += kTPB) {
+      double v = 0.0;
+      // four children at a time: the map lookups, then the entries, all in
+      // flight together; the sum stays in child order
+      for (int t0 = 0; t0 < nl; t0 += 4) {
+        int ri[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int r = child_row_nb(P, s_cv[s_list[jj][min(t0 + u, nl - 1)]], i, ns, din);
+          ri[u] = t0 + u < nl ? r : -1;
+        }
+        T x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int t = min(t0 + u, nl - 1);
+          const ChildView<T>& cv = s_cv[s_list[jj][t]];
+          const int a = max(ri[u], 0), rj = s_rj[jj][t];
+          x[u] = cv.F[max(a, rj) + (size_t)min(a, rj) * cv.m];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (ri[u] >= 0) v += (double)x[u];
+      }
+      cj[i] = (T)v;
+    }
+  }
+  __syncthreads();
+  // original entries (distinct positions), then Sigma, as k_assemble adds them
+  if (oe_u >= 0) {
+    T* c = colv + (size_t)(oe_j - j0) * ldc + (oe_u < ns ? oe_u : oe_u + din);
+    *c = (T)((double)*c + oe_v);
+  }
+  for (int e = e0 + kTPB + tid; e < e1; e += kTPB) {
+    const int u = P.am_row[e], j = P.am_col[e];
+    T* c = colv + (size_t)(j - j0) * ldc + (u < ns ? u : u + din);
+    *c = (T)((double)*c + P.vals[P.am_src[e]]);
+  }
+  __syncthreads();
+  if (P.diag && tid < kAsmLdsCols && j0 + tid < min(j1, ns)) {
+    T* c = colv + (size_t)tid * ldc + j0 + tid;
+    *c = (T)((double)*c + dg);
+  }
+  __syncthreads();
+  T* F = front_ptr<T>(P, p);
+  for (int jj = 0; jj < j1 - j0; jj++) {
+    const int j = j0 + jj;
+    for (int i = j + tid; i < m; i += kTPB) F[(size_t)j * m + i] = colv[(size_t)jj * ldc + i];
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_factor(DevPlan P, const int* nodes) {
   constexpr int NT = kTPB;
@@ -1599,6 +1784,8 @@ struct Handle {
   std::vector<int> tk_lds_ev, tk_wait;
   std::vector<int> tk_chain_end;
   std::vector<int> tk_prep;  // per tick: k_prepare launched (else prepared inline)
+  std::vector<int> tk_asm_cap;  // per tick: largest slot assembled in HBM
+  std::vector<int> tk_asm_lds;  // per tick: assembled by k_assemble_lds
   // LDS fronts take the right-looking kernels of kkt_front.hip (every mode
   // but Cholesky, which keeps k_front_lds)
   bool rl_fronts = true;
@@ -1801,15 +1988,38 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   for (int t = 0; t < T; t++)
     std::sort(H.tk_lds_nodes.begin() + H.tk_lds_ptr[t], H.tk_lds_nodes.begin() + H.tk_lds_ptr[t + 1],
               [&](int a, int b) { return H.cap[a] < H.cap[b]; });
-  // assembly work items of the fronts assembled in HBM, by start tick
+  // k_prepare runs at a tick only where a node needs it before its factor
+  // kernel: Cholesky small fronts, LDS fronts of the blocked kernel, fronts
+  // with more children than k_assemble's LDS cache.  Otherwise k_assemble
+  // and the right-looking LDS kernels prepare their nodes themselves.
+  H.tk_prep.assign(T, 0);
+  for (int t = 0; t < T; t++) {
+    int need = !H.rl_fronts || H.tk_small_ptr[t + 1] > H.tk_small_ptr[t];
+    for (int q = H.tk_grp_ptr[t]; q < H.tk_grp_ptr[t + 1]; q++) {
+      const int k = H.tk_grp_nodes[q];
+      need |= S.children_ptr[k + 1] - S.children_ptr[k] > kChildCache;
+    }
+    H.tk_prep[t] = need;
+  }
+  // assembly work items of the fronts assembled in HBM, by start tick:
+  // kAsmLdsCols columns per item where the tick's fronts are assembled
+  // through LDS (k_assemble_lds), kAsmCols where k_assemble runs
   H.asm_node.clear();
   H.asm_chunk.clear();
   H.tk_asm_ptr.assign(T + 1, 0);
+  H.tk_asm_cap.assign(T, 0);
+  H.tk_asm_lds.assign(T, 0);
   for (int t = 0; t < T; t++) {
     for (int q = H.tk_grp_ptr[t]; q < H.tk_grp_ptr[t + 1]; q++) {
       const int k = H.tk_grp_nodes[q];
-      if (H.cap[k] <= H.lds_front) continue;  // assembled inside k_front_lds
-      const int nch = (H.cap[k] + kAsmCols - 1) / kAsmCols;
+      if (H.cap[k] > H.lds_front) H.tk_asm_cap[t] = std::max(H.tk_asm_cap[t], H.cap[k]);
+    }
+    H.tk_asm_lds[t] = !H.tk_prep[t] && H.tk_asm_cap[t] <= kAsmLdsRows;
+    const int width = H.tk_asm_lds[t] ? kAsmLdsCols : kAsmCols;
+    for (int q = H.tk_grp_ptr[t]; q < H.tk_grp_ptr[t + 1]; q++) {
+      const int k = H.tk_grp_nodes[q];
+      if (H.cap[k] <= H.lds_front) continue;  // assembled inside the LDS front kernels
+      const int nch = (H.cap[k] + width - 1) / width;
       for (int c = 0; c < nch; c++) {
         H.asm_node.push_back(k);
         H.asm_chunk.push_back(c);
@@ -1878,19 +2088,6 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
           H.tk_wait[t] = std::max(H.tk_wait[t], H.tk_lds_ev[start[ch]]);
       }
     }
-  // k_prepare runs at a tick only where a node needs it before its factor
-  // kernel: Cholesky small fronts, LDS fronts of the blocked kernel, fronts
-  // with more children than k_assemble's LDS cache.  Otherwise k_assemble
-  // and the right-looking LDS kernels prepare their nodes themselves.
-  H.tk_prep.assign(T, 0);
-  for (int t = 0; t < T; t++) {
-    int need = !H.rl_fronts || H.tk_small_ptr[t + 1] > H.tk_small_ptr[t];
-    for (int q = H.tk_grp_ptr[t]; q < H.tk_grp_ptr[t + 1]; q++) {
-      const int k = H.tk_grp_nodes[q];
-      need |= S.children_ptr[k + 1] - S.children_ptr[k] > kChildCache;
-    }
-    H.tk_prep[t] = need;
-  }
   // Chains: runs of consecutive ticks whose only work is one LDS front
   // (threshold or Bunch-Kaufman mode) are factored by one workgroup in one
   // launch (k_front_chain); tk_chain_end[t] is the run's last tick at its
@@ -2394,8 +2591,14 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       }
       if (a1 > a0) {
         ProfScope ps_(H, st, KC_ASSEMBLE);
-        hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
-                           H.d_asm_node.p + a0, H.d_asm_chunk.p + a0, H.tk_prep[t] ? 0 : 1);
+        const int ldc = H.tk_asm_cap[t];
+        if (H.tk_asm_lds[t])
+          hipLaunchKernelGGL(k_assemble_lds<T>, dim3(a1 - a0), dim3(kTPB),
+                             sizeof(T) * kAsmLdsCols * (size_t)ldc, st, P, H.d_asm_node.p + a0,
+                             H.d_asm_chunk.p + a0, ldc);
+        else
+          hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
+                             H.d_asm_node.p + a0, H.d_asm_chunk.p + a0, H.tk_prep[t] ? 0 : 1);
       }
       if (s1 > s0) {
         ProfScope ps_(H, st, KC_FACTOR);
