@@ -624,12 +624,33 @@ __global__ void __launch_bounds__(NT, 4) k_front_lds(DevPlan P, const int* nodes
 // D[4(l/16) + r][l%16] (also checked on gfx950); FP64 fronts
 // v_mfma_f64_16x16x4f64, whose output is D[l/16 + 4r][l%16].
 template <class T>
-__global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* wnode,
-                                                 const int* wtile, const int* wbase) {
+__global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* unode,
+                                                 const int* ubase, const int* upref,
+                                                 int nfront) {
   typedef T vec4 __attribute__((ext_vector_type(4)));
-  const int p = wnode[blockIdx.x];
-  const int t = wtile[blockIdx.x];
-  const int base = wbase[blockIdx.x];
+  // the workgroup's front: the last f whose first tile upref[f] is at most
+  // blockIdx.x (the prefixes increase); every wave finds it by a ballot over
+  // 64 fronts at a time, so the result is uniform without a barrier
+  const int g = blockIdx.x;
+  int p = 0, base = 0, t = 0;
+  {
+    const int ln = threadIdx.x % kWave;
+    for (int f0 = 0; f0 < nfront; f0 += kWave) {
+      const int q = min(f0 + ln, nfront - 1);
+      const int pr = upref[q], nd = unode[q], bs = ubase[q];
+      const unsigned long long b = __ballot(f0 + ln < nfront && pr <= g);
+      if (b == 0ull) break;
+      const int src = 63 - __clzll((long long)b);
+      p = __shfl(nd, src);
+      base = __shfl(bs, src);
+      t = g - __shfl(pr, src);
+      if (b != ~0ull) break;
+    }
+    // uniform values: the node's metadata then load through the scalar cache
+    p = __builtin_amdgcn_readfirstlane(p);
+    base = __builtin_amdgcn_readfirstlane(base);
+    t = __builtin_amdgcn_readfirstlane(t);
+  }
   // everything per node depends on p alone: one round of loads
   const int stat = P.stat[p];
   const PanelState st = P.ps[p];
@@ -1433,6 +1454,8 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
 // (4 per wave, lanes over the rows, coalesced), with the node's x rows
 // staged in LDS.
 constexpr int kGemvCols = 16;
+// RB right-hand sides per workgroup with their x rows staged in LDS; RB = 0:
+// one right-hand side, x read directly (rows below the pivots beyond LDS).
 template <class T, int RB>
 __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
                                                    const int* wtile, const double* x,
@@ -1449,22 +1472,26 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const T* L = front_ptr<T>(P, p);
   const int nr = m - npiv;
-  const int r0 = blockIdx.y * RB;
-  const int nb = min(nrhs - r0, RB);
+  constexpr int RR = RB > 0 ? RB : 1;
+  const int r0 = blockIdx.y * RR;
+  const int nb = min(nrhs - r0, RR);
   // both index loads issued (clamped into their arrays; a node without
   // contribution rows reads vars twice), then the gather
   const int cs = P.cs[p];
   const int* crow = cs > 0 ? P.contrib_rows + cp : vars;
-  for (int q = tid; q < nr; q += kTPB) {
+  auto xrow = [&](int q, int rr) {  // x of row npiv + q, right-hand side r0 + rr
     const int i = npiv + q;
     const int iv = vars[min(i, fs - 1)];
     const int ic = crow[min(max(i - fs, 0), max(cs - 1, 0))];
-    const int v = i < fs ? iv : ic;
+    return x[(size_t)(r0 + rr) * ldx + (i < fs ? iv : ic)];
+  };
+  if constexpr (RB > 0) {
+    for (int q = tid; q < nr; q += kTPB)
 #pragma unroll
-    for (int rr = 0; rr < RB; rr++)
-      if (rr < nb) xs[rr * ldxs + q] = x[(size_t)(r0 + rr) * ldx + v];
+      for (int rr = 0; rr < RB; rr++)
+        if (rr < nb) xs[rr * ldxs + q] = xrow(q, rr);
+    __syncthreads();
   }
-  __syncthreads();
   // the wave's four columns together, so that their loads are in flight
   // at once (columns past npiv read column npiv - 1 and are not written);
   // each column accumulates over q = lane, lane + 64, ... in order, for
@@ -1474,11 +1501,12 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   const T* Lc[CW];
 #pragma unroll
   for (int jj = 0; jj < CW; jj++) Lc[jj] = L + (size_t)min(jb + jj, npiv - 1) * m + npiv;
-  double sc[CW][RB];
+  double sc[CW][RR];
 #pragma unroll
   for (int jj = 0; jj < CW; jj++)
 #pragma unroll
-    for (int rr = 0; rr < RB; rr++) sc[jj][rr] = 0.0;
+    for (int rr = 0; rr < RR; rr++) sc[jj][rr] = 0.0;
+  auto xv = [&](int q, int rr) { return RB > 0 ? xs[rr * ldxs + q] : xrow(q, rr); };
   int q = lane;
   for (; q + kWave < nr; q += 2 * kWave) {
     T a[CW][2];
@@ -1488,8 +1516,8 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
       a[jj][1] = Lc[jj][q + kWave];
     }
 #pragma unroll
-    for (int rr = 0; rr < RB; rr++) {
-      const double x0 = xs[rr * ldxs + q], x1 = xs[rr * ldxs + q + kWave];
+    for (int rr = 0; rr < RR; rr++) {
+      const double x0 = xv(q, rr), x1 = xv(q + kWave, rr);
 #pragma unroll
       for (int jj = 0; jj < CW; jj++) {
         sc[jj][rr] = __builtin_fma((double)a[jj][0], x0, sc[jj][rr]);
@@ -1499,8 +1527,8 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   }
   if (q < nr) {
 #pragma unroll
-    for (int rr = 0; rr < RB; rr++) {
-      const double x0 = xs[rr * ldxs + q];
+    for (int rr = 0; rr < RR; rr++) {
+      const double x0 = xv(q, rr);
 #pragma unroll
       for (int jj = 0; jj < CW; jj++)
         sc[jj][rr] = __builtin_fma((double)Lc[jj][q], x0, sc[jj][rr]);
@@ -1509,14 +1537,14 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
 #pragma unroll
   for (int jj = 0; jj < CW; jj++)
 #pragma unroll
-    for (int rr = 0; rr < RB; rr++)
+    for (int rr = 0; rr < RR; rr++)
       for (int off = kWave / 2; off > 0; off >>= 1) sc[jj][rr] += __shfl_xor(sc[jj][rr], off);
   // every lane holds the sums; lane rr * CW + jj writes column jj of
   // right-hand side rr
-  static_assert(RB * CW <= kWave, "one lane per (right-hand side, column)");
+  static_assert(RR * CW <= kWave, "one lane per (right-hand side, column)");
   double v = 0.0;
 #pragma unroll
-  for (int rr = 0; rr < RB; rr++)
+  for (int rr = 0; rr < RR; rr++)
 #pragma unroll
     for (int jj = 0; jj < CW; jj++)
       if (lane == rr * CW + jj) v = sc[jj][rr];
@@ -1757,6 +1785,7 @@ struct Handle {
   std::vector<int> asm_node, asm_chunk;
   long long f_total = 0, i_total = 0, w_total = 0, u_total = 0;
   int max_cap = 0;
+  int max_below = 0;  // largest cap - npiv over the large fronts (last factorization)
   // device
   DBuf<int> d_ns, d_cs, d_cap, d_vptr, d_cptr, d_chptr, d_ch, d_mode, d_n2v,
       d_crows, d_invnom, d_fwdnom, d_amptr, d_amsrc, d_amrow, d_amcol, d_asm_node,
@@ -1795,9 +1824,12 @@ struct Handle {
   std::vector<hipEvent_t> lds_ev;
   std::vector<int> tk_grp_ptr, tk_grp_nodes, tk_lds_ptr, tk_lds_nodes, tk_small_ptr,
       tk_small_nodes, tk_asm_ptr, tk_act_ptr, tk_act_nodes, tk_act_cap, tk_act_root;
-  std::vector<int> upd_ptr, upd_node, upd_tile, upd_base;
+  // per tick t, fronts upd_ptr[t] .. upd_ptr[t + 1] - 1 of the update launch:
+  // node, first column of its tile grid, its first workgroup (exclusive
+  // prefix of the tiles); tk_upd_tiles[t] workgroups in all
+  std::vector<int> upd_ptr, upd_node, upd_pref, upd_base, tk_upd_tiles;
   DBuf<int> d_tk_grp_nodes, d_tk_lds_nodes, d_tk_small_nodes, d_tk_act_nodes, d_upd_node,
-      d_upd_tile, d_upd_base;
+      d_upd_pref, d_upd_base;
   // solve: per level, the nodes solved by one workgroup each (sv), the
   // backward L21^T GEMV tiles of the large nodes (bg, 16 columns each) and
   // their panel steps (fs forward, bs backward: fs_lvl[l] is the first step
@@ -2039,9 +2071,11 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     H.tk_act_root.assign(T, 0);
     H.upd_ptr.assign(T + 1, 0);
     H.upd_node.clear();
-    H.upd_tile.clear();
+    H.upd_pref.clear();
     H.upd_base.clear();
+    H.tk_upd_tiles.assign(T, 0);
     for (int t = 0; t < T; t++) {
+      int tiles = 0;
       for (int k : act[t]) {
         H.tk_act_nodes.push_back(k);
         H.tk_act_cap[t] = std::max(H.tk_act_cap[t], H.cap[k]);
@@ -2050,15 +2084,15 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
         // (one short panel allowed for a mode-3 root)
         const int tp = t - start[k] - (H.mode[k] == 3 ? 1 : 0);
         const int base = std::max(tp, 0) * (kNBLarge - 2);
-        const int nt = (H.cap[k] - base + 63) / 64;
-        for (int tt = 0; tt < nt * (nt + 1) / 2; tt++) {
-          H.upd_node.push_back(k);
-          H.upd_tile.push_back(tt);
-          H.upd_base.push_back(base);
-        }
+        const int nt = std::max((H.cap[k] - base + 63) / 64, 0);
+        H.upd_node.push_back(k);
+        H.upd_base.push_back(base);
+        H.upd_pref.push_back(tiles);
+        tiles += nt * (nt + 1) / 2;
       }
       H.tk_act_ptr[t + 1] = (int)H.tk_act_nodes.size();
       H.upd_ptr[t + 1] = (int)H.upd_node.size();
+      H.tk_upd_tiles[t] = tiles;
     }
   }
   // Side-stream events: one per tick that launches LDS fronts, recorded
@@ -2362,7 +2396,7 @@ static void upload_static(Handle& H) {
   H.d_tk_act_nodes.upload(H.tk_act_nodes);
   H.d_large_nodes.upload(H.large_nodes);
   H.d_upd_node.upload(H.upd_node);
-  H.d_upd_tile.upload(H.upd_tile);
+  H.d_upd_pref.upload(H.upd_pref);
   H.d_upd_base.upload(H.upd_base);
   H.d_sv_nodes.upload(H.sv_nodes);
   H.d_bg_node.upload(H.bg_node);
@@ -2616,11 +2650,11 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
                            H.d_tk_act_nodes.p + g0);
       }
       const int u0 = H.upd_ptr[t], u1 = H.upd_ptr[t + 1];
-      if (u1 > u0) {
+      if (H.tk_upd_tiles[t] > 0) {
         ProfScope ps_(H, st, KC_UPDATE, true);
-        hipExtLaunchKernelGGL(k_update<T>, dim3(u1 - u0), dim3(kTPB), 0, st, ps_.start(),
-                              ps_.stop(), 0, P, H.d_upd_node.p + u0, H.d_upd_tile.p + u0,
-                              H.d_upd_base.p + u0);
+        hipExtLaunchKernelGGL(k_update<T>, dim3(H.tk_upd_tiles[t]), dim3(kTPB), 0, st,
+                              ps_.start(), ps_.stop(), 0, P, H.d_upd_node.p + u0,
+                              H.d_upd_base.p + u0, H.d_upd_pref.p + u0, u1 - u0);
       }
     }
   }
@@ -2685,9 +2719,8 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
   ensure_device(H);
   // the large-front panel (vars/pre) and the solve's front gather and
   // L21^T products stage a front's rows in LDS (two int or one double per row)
-  if ((size_t)((H.max_cap + 31) & ~31) * sizeof(double) > 160 * 1024)
-    throw Error(-6, "front of " + std::to_string(H.max_cap) +
-                        " rows exceeds the 20448 rows the LDS-staged panel and solve kernels hold");
+  // (fronts of any size: the kernels whose LDS staging grows with the front
+  // have global-memory forms for the fronts beyond LDS)
   const hipMemcpyKind kind = mem == KKT_LDL_DEVICE ? hipMemcpyDeviceToDevice
                                                    : hipMemcpyHostToDevice;
   if (vals) {
@@ -2753,6 +2786,9 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
   HIPCHECK(hipMemcpy(H.h_npiv.data(), H.d_npiv.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(H.h_ndel.data(), H.d_ndel.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(H.h_cnt.data(), H.d_cnt.p, sizeof(int) * 5 * H.N, hipMemcpyDeviceToHost));
+  // rows below the pivots of the large fronts (the backward GEMV's x staging)
+  H.max_below = 0;
+  for (int k : H.large_nodes) H.max_below = std::max(H.max_below, H.cap[k] - H.h_npiv[k]);
   // the solve launches only the panel steps that hold pivots
   const int nlev = (int)H.lvl_ptr.size() - 1;
   H.lvl_steps.assign(nlev, 0);
@@ -2804,7 +2840,7 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     return Q;
   };
   size_t lds = 0;
-  const size_t lds_bg = sizeof(double) * (size_t)H.max_cap;
+  const size_t lds_bg = sizeof(double) * (size_t)std::max(H.max_below, 1);
   const long long ys = H.i_total, us = H.u_total;
   double* Y = H.d_Y.p;
   constexpr int kGatherThreads = 1024;
@@ -2824,9 +2860,8 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
       if (nrhs == 1)
-        hipLaunchKernelGGL((k_forward<T, kGatherThreads, true>), dim3(g1 - g0, nrhs),
-                           dim3(kGatherThreads), sizeof(double) * (size_t)lds_t, st, P,
-                           H.d_large_nodes.p + g0, dx, ldx, nrhs, us, chol, Y, ys);
+        hipLaunchKernelGGL((k_gather_big<T, 1>), dim3(g1 - g0, 1), dim3(kGatherThreads), 0, st, P,
+                           H.d_large_nodes.p + g0, dx, ldx, nrhs, us, Y, ys);
       else
         hipLaunchKernelGGL((k_gather_big<T, kGatherRhs>),
                            dim3(g1 - g0, (nrhs + kGatherRhs - 1) / kGatherRhs), dim3(kGatherThreads),
@@ -2852,10 +2887,15 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     if (g1 > g0) {
       const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
       // one right-hand side per workgroup (batching them here serialises a
-      // workgroup's dot products and was measured slower)
-      hipLaunchKernelGGL((k_bwd_gemv<T, 1>), dim3(b1 - b0, nrhs), dim3(kTPB), lds_bg, st, P,
-                         H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys,
-                         H.max_cap);
+      // workgroup's dot products and was measured slower); the x rows below
+      // the pivots staged in LDS where they fit
+      if (lds_bg <= 128 * 1024)
+        hipLaunchKernelGGL((k_bwd_gemv<T, 1>), dim3(b1 - b0, nrhs), dim3(kTPB), lds_bg, st, P,
+                           H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys,
+                           H.max_below);
+      else
+        hipLaunchKernelGGL((k_bwd_gemv<T, 0>), dim3(b1 - b0, nrhs), dim3(kTPB), 0, st, P,
+                           H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys, 0);
       const int nst = std::min(H.bs_lvl[l + 1] - H.bs_lvl[l], H.lvl_steps[l]);
       for (int t = 0; t < nst; t++) {
         const int slot = H.bs_lvl[l] + t;

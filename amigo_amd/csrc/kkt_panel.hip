@@ -14,7 +14,9 @@ namespace kktldl {
 // Large fronts: one panel per launch on one workgroup (state in P.ps), the
 // trailing update spread over many workgroups (k_update).
 // --------------------------------------------------------------------------
-template <int NT, class T>
+// GV: the front's vars and pre stay in HBM (fronts whose bookkeeping does
+// not fit LDS); otherwise they are staged in LDS for the panel.
+template <int NT, class T, bool GV>
 __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
   extern __shared__ int s_iv[];  // vars and pre of the front, staged in LDS
   const int p = nodes[blockIdx.x];
@@ -43,20 +45,22 @@ __global__ void __launch_bounds__(NT) k_panel(DevPlan P, const int* nodes) {
     const int mode = P.mode[p] == 3 ? bkphase : P.mode[p];
     const int nb = max(2, min(kNBLarge, m));
     const long long io = P.ioff[p];
-    int* vars = s_iv;
-    int* pre = s_iv + m;
-    for (int i = k + tid; i < m; i += NT) {
-      vars[i] = P.vars[io + i];
-      pre[i] = P.pre[io + i];
-    }
+    int* vars = GV ? P.vars + io : s_iv;
+    int* pre = GV ? P.pre + io : s_iv + m;
+    if (!GV)
+      for (int i = k + tid; i < m; i += NT) {
+        vars[i] = P.vars[io + i];
+        pre[i] = P.pre[io + i];
+      }
     __syncthreads();
     panel_factor<NT, T>(front_ptr<T>(P, p), m, m, vars, pre, P.ptype + io, P.dv + io,
                         P.dv2 + io, P.W + P.woff[p], P.cap[p], nb, mode, P.ustab, P.ptol,
                         k, nc, kw, sh);
-    for (int i = kstart + tid; i < m; i += NT) {
-      P.vars[io + i] = vars[i];
-      P.pre[io + i] = pre[i];
-    }
+    if (!GV)
+      for (int i = kstart + tid; i < m; i += NT) {
+        P.vars[io + i] = vars[i];
+        P.pre[io + i] = pre[i];
+      }
     if (tid == 0 && k >= nc) {
       const int fs = P.ns[p] + P.din[p];
       P.npiv[p] = nc;
@@ -985,9 +989,15 @@ bool launch_large_panel(const KKT_PANEL_T*, bool bkc, int maxcap, dim3 grid, hip
   using T = KKT_PANEL_T;
   const bool rr = bkc ? launch_panel_rr<T, true>(maxcap, grid, st, e0, e1, P, nodes)
                       : launch_panel_rr<T, false>(maxcap, grid, st, e0, e1, P, nodes);
-  if (!rr)
-    hipExtLaunchKernelGGL((k_panel<kPanelThreads, T>), grid, dim3(kPanelThreads),
-                          (uint32_t)(sizeof(int) * 2 * (size_t)maxcap), st, e0, e1, 0, P, nodes);
+  if (!rr) {
+    const size_t iv = sizeof(int) * 2 * (size_t)maxcap;
+    if (iv <= 96 * 1024)
+      hipExtLaunchKernelGGL((k_panel<kPanelThreads, T, false>), grid, dim3(kPanelThreads),
+                            (uint32_t)iv, st, e0, e1, 0, P, nodes);
+    else
+      hipExtLaunchKernelGGL((k_panel<kPanelThreads, T, true>), grid, dim3(kPanelThreads), 0, st,
+                            e0, e1, 0, P, nodes);
+  }
   return rr;
 }
 
