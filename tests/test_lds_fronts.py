@@ -146,3 +146,22 @@ def test_chain_overflow_replans():
     x = P.rhs.copy()
     s.solve(x, refine=True)
     assert _relres(P.matrix(), x, P.rhs) <= 1e-10
+
+
+def test_elasticity_fronts_beyond_lds_bookkeeping():
+    """30^3 elasticity KKT (n = 268,119) in FP32 + refinement: its largest
+    front (13,392 rows) takes the global-memory panel with the front's
+    vars/pre kept in HBM (they do not fit LDS beside the panel), the others
+    the LDS panels; inertia equal to the structural one (every row positive
+    except the free multipliers) and a refined residual <= 1e-10."""
+    from amigo_amd.ldl import FP32_REFINE
+    P = W.elasticity3d(N=30)
+    s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
+                  order=OrderingType.NESTED_DISSECTION, precision=FP32_REFINE, max_refine=30)
+    assert s.stats()["max_front"] > 12288
+    assert s.factor(P.diag) == 0
+    nd = len(P.dual_idx)
+    assert s.get_inertia() == (P.n - nd, nd)
+    x = P.rhs.copy()
+    assert s.solve(x, refine=True) == 0
+    assert _relres(P.matrix(), x, P.rhs) <= 1e-10
