@@ -72,8 +72,7 @@ struct DevPlan {
   const int* am_src;   // CSR index into vals
   const int* am_row;   // nominal row in the front
   const int* am_col;   // own column of the entry inside its front
-  int lds_t;           // solve kernels: doubles of LDS before the staged block
-  int lds_b;           // solve kernels: leading dimension of the staged block
+  int lds_t;           // small-front solve kernels: doubles of LDS (the front vector)
   // dynamic per node
   int* din;            // delayed pivots received
   int* m;              // actual front dimension

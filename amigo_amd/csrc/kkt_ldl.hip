@@ -778,13 +778,13 @@ __device__ inline int block_end(const int* ptype, int jb, int npiv, int chol) {
 // BIG nodes (the large fronts, few per level) only gather here: the
 // pivot rows of the front vector go to Y and the rows below to U, and the
 // panel steps (k_fwd_step) solve them over many workgroups.
+// One node p for right-hand side r: front vector t (LDS, P.lds_t doubles),
+// child views s_cv / s_uo (LDS, kChildCache).
+// Ends with a barrier, so that a workgroup can go on to another node.
 template <class T, int NTH, bool BIG>
-__global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
-                                                 double* x, int ldx, int nrhs,
-                                                 long long ustride, int chol, double* Y,
-                                                 long long ystride) {
-  extern __shared__ double t[];
-  const int p = nodes[blockIdx.x];
+__device__ __attribute__((always_inline)) inline void forward_node(
+    const DevPlan& P, int p, int r, double* x, int ldx, long long ustride, int chol, double* Y,
+    long long ystride, double* t, ChildView<T>* s_cv, long long* s_uo) {
   const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const int m = P.m[p], npiv = P.npiv[p];
   const int ns = P.ns[p], din = P.din[p];
@@ -797,8 +797,6 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
   const double* dv2 = P.dv2 + P.ioff[p];
   const int c0 = P.chptr[p], c1 = P.chptr[p + 1];
   const int vp = P.vptr[p];
-  __shared__ ChildView<T> s_cv[kChildCache];
-  __shared__ long long s_uo[kChildCache];
   const int nch = c1 - c0;
   const bool cached = nch <= kChildCache;
   if (cached)
@@ -808,7 +806,6 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
     }
   __syncthreads();
   {
-    const int r = blockIdx.y;  // one right-hand side per grid row
     double* xr = x + (size_t)r * ldx;
     for (int i = tid; i < m; i += NTH) {
       const int pi = i < fs ? pre[i] : i;
@@ -853,34 +850,35 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
         if (i < npiv) y[i] = t[i];
         else U[i - npiv] = t[i];
       }
+      __syncthreads();
       return;
     }
     for (int jb = 0; jb < npiv;) {
       const int je = block_end(ptype, jb, npiv, chol);
       const int bs = je - jb;
-      const int ldb = P.lds_b;
-      double* Lb = t + P.lds_t;  // bs x bs block, column major, leading dimension ldb
-      for (int e = tid; e < bs * bs; e += NTH) {
-        const int c = e / bs, rr = e - c * bs;
-        Lb[rr + c * ldb] = (double)L[jb + rr + (size_t)(jb + c) * m];
-      }
-      __syncthreads();
       if (w == 0) {
+        // one wave, lane = row of the block: the row's block entries in
+        // registers (one round of loads from clamped addresses), then the
+        // column steps with readlane broadcasts
         const int row = jb + lane;
         double tr = lane < bs ? t[row] : 0.0;
         const double dr = (chol && lane < bs) ? dv[row] : 1.0;
         const int pt = (!chol && lane < bs) ? ptype[row] : 1;
-        // c is uniform: the broadcasts are readlanes, not LDS permutes
-        for (int c = 0; c < bs; c++) {
+        const size_t ro = (size_t)min(row, m - 1);
+        double lr[kSB];
+#pragma unroll
+        for (int c = 0; c < kSB; c++) lr[c] = (double)L[ro + (size_t)(jb + min(c, bs - 1)) * m];
+#pragma unroll
+        for (int c = 0; c < kSB; c++) {
+          if (c >= bs) continue;
           const int ty = __builtin_amdgcn_readlane(pt, c);
           if (ty == 1) {
             if (chol && lane == c) tr = tr / dr;
             const double tc = lane_d(tr, c);
-            if (lane > c && lane < bs) tr -= Lb[lane + c * ldb] * tc;
+            if (lane > c && lane < bs) tr -= lr[c] * tc;
           } else if (ty == 2) {
             const double ta = lane_d(tr, c), tb = lane_d(tr, c + 1);
-            if (lane > c + 1 && lane < bs)
-              tr -= Lb[lane + c * ldb] * ta + Lb[lane + (c + 1) * ldb] * tb;
+            if (lane > c + 1 && lane < bs) tr -= lr[c] * ta + lr[min(c + 1, kSB - 1)] * tb;
           }
         }
         if (lane < bs) t[row] = tr;
@@ -914,15 +912,83 @@ __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
   }
 }
 
+template <class T, int NTH, bool BIG>
+__global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
+                                                 double* x, int ldx, int nrhs,
+                                                 long long ustride, int chol, double* Y,
+                                                 long long ystride) {
+  extern __shared__ double t[];
+  __shared__ ChildView<T> s_cv[kChildCache];
+  __shared__ long long s_uo[kChildCache];
+  // one node per workgroup, one right-hand side per grid row
+  forward_node<T, NTH, BIG>(P, nodes[blockIdx.x], blockIdx.y, x, ldx, ustride, chol, Y, ystride,
+                            t, s_cv, s_uo);
+}
+
+// A run of levels holding one small node each (a chain: each node's
+// children are at earlier levels): one workgroup per right-hand side walks
+// the nodes in level order, one launch for the run.
+template <class T, int NTH>
+__global__ void __launch_bounds__(NTH) k_forward_chain(DevPlan P, const int* nodes, int count,
+                                                       double* x, int ldx, long long ustride,
+                                                       int chol) {
+  extern __shared__ double t[];
+  __shared__ ChildView<T> s_cv[kChildCache];
+  __shared__ long long s_uo[kChildCache];
+  for (int q = 0; q < count; q++)
+    forward_node<T, NTH, false>(P, nodes[q], blockIdx.y, x, ldx, ustride, chol, nullptr, 0, t,
+                                s_cv, s_uo);
+}
+
 // Backward: t1 = x[pivots] - L21^T x[rows below], then the L11^T back solve
 // in blocks from the bottom (dot products over the rows below a block by
 // the whole workgroup, the diagonal block by one wave in registers).
 // BIG nodes are solved by k_bwd_gemv and the panel steps (k_bwd_step).
+// t[j] -= sum_{i0 <= i < i1} L[i, j] t[i] for the columns j0 <= j < j1:
+// one wave per column, lanes over i in steps of kWave (strided_dot's order),
+// then a butterfly.  With at most two terms per lane, CB columns of a wave
+// have their loads in flight together.
 template <class T, int NTH>
-__global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
-                                                  double* x, int ldx, int nrhs, int chol) {
-  extern __shared__ double t[];
-  const int p = nodes[blockIdx.x];
+__device__ __attribute__((always_inline)) inline void col_dots(const T* L, int m, double* t,
+                                                               int j0, int j1, int i0, int i1) {
+  constexpr int NW = NTH / kWave;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (i1 - i0 <= 2 * kWave) {
+    constexpr int CB = 8;
+    const int ia = i0 + lane, ib = ia + kWave;
+    const int ra = min(ia, max(i1 - 1, 0)), rb = min(ib, max(i1 - 1, 0));
+    const double t0 = ia < i1 ? t[ia] : 0.0, t1 = ib < i1 ? t[ib] : 0.0;
+    for (int jc = j0 + w; jc < j1; jc += CB * NW) {
+      double a0[CB], a1[CB];
+#pragma unroll
+      for (int u = 0; u < CB; u++) {
+        const size_t co = (size_t)min(jc + u * NW, j1 - 1) * m;
+        a0[u] = (double)L[ra + co];
+        a1[u] = (double)L[rb + co];
+      }
+#pragma unroll
+      for (int u = 0; u < CB; u++) {
+        double s = 0.0;
+        if (ia < i1) s += a0[u] * t0;
+        if (ib < i1) s += a1[u] * t1;
+        for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        const int j = jc + u * NW;
+        if (lane == 0 && j < j1) t[j] -= s;
+      }
+    }
+  } else {
+    for (int j = j0 + w; j < j1; j += NW) {
+      double s = strided_dot(L + (size_t)j * m, t, i0 + lane, i1, kWave);
+      for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if (lane == 0) t[j] -= s;
+    }
+  }
+}
+
+template <class T, int NTH>
+__device__ __attribute__((always_inline)) inline void backward_node(const DevPlan& P, int p, int r,
+                                                                    double* x, int ldx, int chol,
+                                                                    double* t) {
   const int tid = threadIdx.x;
   const int m = P.m[p], npiv = P.npiv[p];
   const int fs = P.ns[p] + P.din[p];
@@ -933,19 +999,14 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
   const int cp = P.cptr[p];
   const int w = tid / kWave, lane = tid % kWave;
   {
-    const int r = blockIdx.y;  // one right-hand side per grid row
     double* xr = x + (size_t)r * ldx;
     for (int i = tid; i < m; i += NTH) {
       int v = i < npiv ? unpiv(vars[i]) : (i < fs ? vars[i] : P.contrib_rows[cp + i - fs]);
       t[i] = xr[v];
     }
     __syncthreads();
-    // t[j] -= sum_{i >= npiv} L[i,j] t[i]  (one wave per column)
-    for (int j = w; j < npiv; j += NTH / kWave) {
-      double s = strided_dot(L + (size_t)j * m, t, npiv + lane, m, kWave);
-      for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) t[j] -= s;
-    }
+    // t[j] -= sum_{i >= npiv} L[i,j] t[i]
+    col_dots<T, NTH>(L, m, t, 0, npiv, npiv, m);
     __syncthreads();
     for (int je = npiv; je > 0;) {
       int jb = max(0, je - kSB);
@@ -953,37 +1014,33 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
       const int bs = je - jb;
       // rows of L11 below the block are final: dot products per column
       if (je < npiv) {
-        for (int j = jb + w; j < je; j += NTH / kWave) {
-          double s = strided_dot(L + (size_t)j * m, t, je + lane, npiv, kWave);
-          for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
-          if (lane == 0) t[j] -= s;
-        }
+        col_dots<T, NTH>(L, m, t, jb, je, je, npiv);
         __syncthreads();
       }
-      const int ldb = P.lds_b;
-      double* Lb = t + P.lds_t;
-      for (int e = tid; e < bs * bs; e += NTH) {
-        const int c = e / bs, rr = e - c * bs;
-        Lb[rr + c * ldb] = (double)L[jb + rr + (size_t)(jb + c) * m];
-      }
-      __syncthreads();
       if (w == 0) {
-        // lane c owns column jb + c of the block
+        // lane c owns column jb + c of the block: the column's block
+        // entries in registers (one round of loads from clamped addresses)
         const int col = jb + lane;
         const double dc = (chol && lane < bs) ? dv[col] : 1.0;
         double tv = lane < bs ? t[col] : 0.0;
         double acc = 0.0;
         const int pt = (!chol && lane < bs) ? ptype[col] : 1;
-        for (int i = bs - 1; i >= 0; i--) {
+        const T* Lc = L + (size_t)min(col, je - 1) * m + jb;
+        double lc[kSB];
+#pragma unroll
+        for (int i = 0; i < kSB; i++) lc[i] = (double)Lc[min(i, bs - 1)];
+#pragma unroll
+        for (int i = kSB - 1; i >= 0; i--) {
+          if (i >= bs) continue;
           const int ty = __builtin_amdgcn_readlane(pt, i);
           if (ty == 1) {
             if (lane == i) tv = chol ? (tv - acc) / dc : tv - acc;
             const double ti = lane_d(tv, i);
-            if (lane < i) acc += Lb[i + lane * ldb] * ti;
+            if (lane < i) acc += lc[i] * ti;
           } else if (ty == 3) {
             if (lane == i || lane == i - 1) tv -= acc;
-            const double ti = lane_d(tv, i), tj = lane_d(tv, i - 1);
-            if (lane < i - 1) acc += Lb[i + lane * ldb] * ti + Lb[i - 1 + lane * ldb] * tj;
+            const double ti = lane_d(tv, i), tj = lane_d(tv, max(i - 1, 0));
+            if (lane < i - 1) acc += lc[i] * ti + lc[max(i - 1, 0)] * tj;
           }
         }
         if (lane < bs) t[col] = tv;
@@ -994,6 +1051,24 @@ __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
     for (int j = tid; j < npiv; j += NTH) xr[unpiv(vars[j])] = t[j];
     __syncthreads();
   }
+}
+
+template <class T, int NTH>
+__global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
+                                                  double* x, int ldx, int nrhs, int chol) {
+  extern __shared__ double t[];
+  // one node per workgroup, one right-hand side per grid row
+  backward_node<T, NTH>(P, nodes[blockIdx.x], blockIdx.y, x, ldx, chol, t);
+}
+
+// The backward sweep of a chain (k_forward_chain): the nodes in reverse
+// level order, parents first.
+template <class T, int NTH>
+__global__ void __launch_bounds__(NTH) k_backward_chain(DevPlan P, const int* nodes, int count,
+                                                        double* x, int ldx, int chol) {
+  extern __shared__ double t[];
+  for (int q = count - 1; q >= 0; q--)
+    backward_node<T, NTH>(P, nodes[q], blockIdx.y, x, ldx, chol, t);
 }
 
 
@@ -1837,8 +1912,16 @@ struct Handle {
   // Y holds a large node's pivot vector
   std::vector<int> sv_ptr, sv_nodes, bg_ptr, bg_node, bg_tile;
   // sv nodes of a level: the first sv_tiny[l] have cap <= kTinyFront and are
-  // solved by one wave each; LDS shape per (level, group): sv_lt, sv_lb
-  std::vector<int> sv_tiny, sv_lt, sv_lb;
+  // solved by one wave each; LDS per (level, group): sv_lt doubles
+  std::vector<int> sv_tiny, sv_lt;
+  // runs of levels holding one small node each and no large node (chains,
+  // e.g. banded KKTs under AMD): the solve walks a run in one launch
+  // (k_forward_chain, k_backward_chain).  sv_run_end[l] / sv_run_start[e]:
+  // the run [l, e] (-1 elsewhere), its LDS doubles sv_run_lt.
+  // KKT_LDL_SOLVE_CHAINS=0 keeps one launch per level (same results, bit
+  // for bit).
+  std::vector<int> sv_run_end, sv_run_start, sv_run_lt;
+  bool solve_chains = true;
   std::vector<int> fs_lvl, fs_ptr, fs_node, fs_tile, bs_lvl, bs_ptr, bs_node, bs_tile;
   DBuf<int> d_sv_nodes, d_bg_node, d_bg_tile, d_fs_node, d_fs_tile, d_bs_node, d_bs_tile;
   DBuf<double> d_Y;
@@ -2180,7 +2263,6 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   H.sv_ptr.assign(nlev + 1, 0);
   H.sv_tiny.assign(nlev, 0);
   H.sv_lt.assign(2 * (size_t)nlev, 0);
-  H.sv_lb.assign(2 * (size_t)nlev, 0);
   H.bg_ptr.assign(nlev + 1, 0);
   H.sv_nodes.clear();
   H.bg_node.clear();
@@ -2229,15 +2311,13 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     H.sv_ptr[l + 1] = (int)H.sv_nodes.size();
     H.bg_ptr[l + 1] = (int)H.bg_node.size();
     // tiny fronts (cap <= kTinyFront) lead the level's list (LDS fronts are
-    // sorted by cap); each group gets LDS for its largest front vector and
-    // its largest diagonal block
+    // sorted by cap); each group gets LDS for its largest front vector
     for (int q = H.sv_ptr[l]; q < H.sv_ptr[l + 1]; q++) {
       const int k = H.sv_nodes[q];
       const bool tiny = H.cap[k] <= kTinyFront && q - H.sv_ptr[l] == H.sv_tiny[l];
       if (tiny) H.sv_tiny[l]++;
       const int g = 2 * l + (tiny ? 0 : 1);
       H.sv_lt[g] = std::max(H.sv_lt[g], (H.cap[k] + 7) & ~7);
-      H.sv_lb[g] = std::max(H.sv_lb[g], std::min(kSB, H.cap[k] - H.cs[k]));
     }
     // forward step t: rows [b1, m) with b1 > t*kPS; tile 0 always exists
     H.fs_lvl[l] = (int)H.fs_ptr.size() - 1;
@@ -2271,6 +2351,29 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   }
   H.fs_lvl[nlev] = (int)H.fs_ptr.size() - 1;
   H.bs_lvl[nlev] = (int)H.bs_ptr.size() - 1;
+  H.sv_run_end.assign(nlev, -1);
+  H.sv_run_start.assign(nlev, -1);
+  H.sv_run_lt.assign(nlev, 0);
+  auto single = [&](int l) {
+    return H.sv_ptr[l + 1] - H.sv_ptr[l] == 1 && H.large_ptr[l + 1] == H.large_ptr[l];
+  };
+  for (int l = 0; l < nlev;) {
+    if (!single(l)) {
+      l++;
+      continue;
+    }
+    int e = l;
+    while (e + 1 < nlev && single(e + 1)) e++;
+    if (e > l) {
+      H.sv_run_end[l] = e;
+      H.sv_run_start[e] = l;
+      int lt = 0;
+      for (int q = H.sv_ptr[l]; q < H.sv_ptr[e + 1]; q++)
+        lt = std::max(lt, (H.cap[H.sv_nodes[q]] + 7) & ~7);
+      H.sv_run_lt[l] = lt;
+    }
+    l = e + 1;
+  }
   H.pinv_total = pv;
 }
 
@@ -2829,13 +2932,11 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   const int chol = H.opt.solver_type == KKT_LDL_CHOLESKY;
   const int lds_t = (H.max_cap + 31) & ~31;
   P.lds_t = lds_t;
-  P.lds_b = kSB;
   // per (level, group) plan copies with their LDS shape
   auto sv_plan = [&](int g, size_t* bytes) {
     DevPlan Q = P;
     Q.lds_t = H.sv_lt[g];
-    Q.lds_b = std::max(1, H.sv_lb[g]);
-    *bytes = sizeof(double) * ((size_t)Q.lds_t + (size_t)Q.lds_b * Q.lds_b);
+    *bytes = sizeof(double) * (size_t)Q.lds_t;
     if (*bytes > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
     return Q;
   };
@@ -2844,8 +2945,25 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   const long long ys = H.i_total, us = H.u_total;
   double* Y = H.d_Y.p;
   constexpr int kGatherThreads = 1024;
+  // a chain's LDS shape and launch (one workgroup per right-hand side)
+  auto chain_plan = [&](int l0, size_t* bytes) {
+    DevPlan Q = P;
+    Q.lds_t = H.sv_run_lt[l0];
+    *bytes = sizeof(double) * (size_t)Q.lds_t;
+    if (*bytes > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
+    return Q;
+  };
   for (int l = 0; l < nlev; l++) {
     ProfScope ps_(H, st, KC_FORWARD);
+    if (H.solve_chains && H.sv_run_end[l] > l) {
+      const int e = H.sv_run_end[l];
+      const DevPlan Q = chain_plan(l, &lds);
+      hipLaunchKernelGGL((k_forward_chain<T, kTPB>), dim3(1, nrhs), dim3(kTPB), lds, st, Q,
+                         H.d_sv_nodes.p + H.sv_ptr[l], H.sv_ptr[e + 1] - H.sv_ptr[l], dx, ldx,
+                         us, chol);
+      l = e;
+      continue;
+    }
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
     if (sm > s0) {
       const DevPlan Q = sv_plan(2 * l, &lds);
@@ -2883,6 +3001,15 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   }
   for (int l = nlev - 1; l >= 0; l--) {
     ProfScope ps_(H, st, KC_BACKWARD);
+    if (H.solve_chains && H.sv_run_start[l] >= 0 && H.sv_run_start[l] < l) {
+      const int b = H.sv_run_start[l];
+      const DevPlan Q = chain_plan(b, &lds);
+      hipLaunchKernelGGL((k_backward_chain<T, kTPB>), dim3(1, nrhs), dim3(kTPB), lds, st, Q,
+                         H.d_sv_nodes.p + H.sv_ptr[b], H.sv_ptr[l + 1] - H.sv_ptr[b], dx, ldx,
+                         chol);
+      l = b;
+      continue;
+    }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
       const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
@@ -3100,6 +3227,7 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   if (const char* e = std::getenv("KKT_LDL_RL_FRONTS")) H.rl_fronts &= std::atoi(e) != 0;
   H.lds_front = std::min(H.lds_front, rl_max_front());
   if (const char* e = std::getenv("KKT_LDL_ROOT_BK")) H.root_bk = std::atoi(e) != 0;
+  if (const char* e = std::getenv("KKT_LDL_SOLVE_CHAINS")) H.solve_chains = std::atoi(e) != 0;
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));
   H.stats.n = n;

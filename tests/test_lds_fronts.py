@@ -165,3 +165,37 @@ def test_elasticity_fronts_beyond_lds_bookkeeping():
     x = P.rhs.copy()
     assert s.solve(x, refine=True) == 0
     assert _relres(P.matrix(), x, P.rhs) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
+def test_chain_solve_bitwise_equal_level_launches(prec):
+    """The AMD cart-pole KKT's solve levels hold one front each: the forward
+    and backward sweeps walk the chain in one launch per right-hand side
+    (k_forward_chain / k_backward_chain) instead of one launch per level
+    (KKT_LDL_SOLVE_CHAINS=0).  Same per-node code: bitwise equal for 1 and
+    3 right-hand sides and for the refined solve, refined residual <= 1e-10."""
+    import torch
+    P = W.cartpole()
+    hs = []
+    for flag in ("1", "0"):
+        os.environ["KKT_LDL_SOLVE_CHAINS"] = flag
+        try:
+            s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals), order=OrderingType.AMD,
+                          precision=prec, max_refine=30)
+        finally:
+            del os.environ["KKT_LDL_SOLVE_CHAINS"]
+        assert s.factor(P.diag) == 0
+        hs.append(s)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    B = torch.randn(3, P.n, dtype=torch.float64, device="cuda", generator=g).t()
+    for nrhs in (1, 3):
+        xa, xb = B[:, :nrhs].clone(), B[:, :nrhs].clone()
+        hs[0].solve(xa)
+        hs[1].solve(xb)
+        assert torch.equal(xa, xb), nrhs
+    xa = torch.as_tensor(P.rhs, device="cuda").clone()
+    xb = xa.clone()
+    assert hs[0].solve(xa, refine=True) == 0
+    assert hs[1].solve(xb, refine=True) == 0
+    assert torch.equal(xa, xb)
+    assert _relres(P.matrix(), xa.cpu().numpy(), P.rhs) <= 1e-10
