@@ -72,7 +72,8 @@ struct DevPlan {
   const int* am_src;   // CSR index into vals
   const int* am_row;   // nominal row in the front
   const int* am_col;   // own column of the entry inside its front
-  int lds_t;           // small-front solve kernels: doubles of LDS (the front vector)
+  int lds_t;           // small-front solve kernels: doubles of LDS before the staged block
+  int lds_b;           // small-front solve kernels: leading dimension of the staged block
   // dynamic per node
   int* din;            // delayed pivots received
   int* m;              // actual front dimension

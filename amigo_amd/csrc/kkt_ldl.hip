@@ -765,6 +765,9 @@ __global__ void __launch_bounds__(kTPB) k_update(DevPlan P, const int* unode,
 // block held in registers, the rows below by the whole workgroup.
 // --------------------------------------------------------------------------
 constexpr int kSB = 64;
+// small-front solve launches of at most this many workgroups take the
+// register-resident diagonal block (REG)
+constexpr int kRegLaunch = 256;
 constexpr int kTinyFront = 64;  // fronts up to this dimension: one wave each
 
 // End of the solve block starting at jb: kSB columns, shortened so that a
@@ -778,13 +781,42 @@ __device__ inline int block_end(const int* ptype, int jb, int npiv, int chol) {
 // BIG nodes (the large fronts, few per level) only gather here: the
 // pivot rows of the front vector go to Y and the rows below to U, and the
 // panel steps (k_fwd_step) solve them over many workgroups.
-// One node p for right-hand side r: front vector t (LDS, P.lds_t doubles),
-// child views s_cv / s_uo (LDS, kChildCache).
+// The diagonal block L[jb:jb+bs, jb:jb+bs] into LDS (column major, leading
+// dimension ldb) as doubles, eight loads of a thread in flight together.
+template <class T, int NTH>
+__device__ __attribute__((always_inline)) inline void stage_block(const T* L, int m, int jb,
+                                                                  int bs, double* Lb, int ldb) {
+  const int n = bs * bs;
+  for (int e0 = threadIdx.x; e0 < n; e0 += 8 * NTH) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = min(e0 + u * NTH, n - 1);
+      const int c = e / bs, rr = e - c * bs;
+      v[u] = (double)L[jb + rr + (size_t)(jb + c) * m];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * NTH;
+      if (e < n) {
+        const int c = e / bs, rr = e - c * bs;
+        Lb[rr + c * ldb] = v[u];
+      }
+    }
+  }
+}
+
+// t[j] -= sum_{i0 <= i < i1} L[i, j] t[i] for the columns j0 <= j < j1:
+// one wave per column, lanes over i in steps of kWave (strided_dot's order),
+// then a butterfly.  With at most two terms per lane, CB columns of a wave
+// have their loads in flight together.
+// One node p for right-hand side r: front vector t (LDS, P.lds_t doubles,
+// then the diagonal block), child views s_cv / s_uo (LDS, kChildCache).
 // Ends with a barrier, so that a workgroup can go on to another node.
-template <class T, int NTH, bool BIG>
+template <class T, int NTH, bool REG>
 __device__ __attribute__((always_inline)) inline void forward_node(
-    const DevPlan& P, int p, int r, double* x, int ldx, long long ustride, int chol, double* Y,
-    long long ystride, double* t, ChildView<T>* s_cv, long long* s_uo) {
+    const DevPlan& P, int p, int r, double* x, int ldx, long long ustride, int chol, double* t,
+    ChildView<T>* s_cv, long long* s_uo) {
   const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const int m = P.m[p], npiv = P.npiv[p];
   const int ns = P.ns[p], din = P.din[p];
@@ -843,42 +875,67 @@ __device__ __attribute__((always_inline)) inline void forward_node(
       t[i] = v;
     }
     __syncthreads();
-    if (BIG) {
-      double* U = P.U + P.uoff[p] + (long long)r * ustride;
-      double* y = Y + (long long)r * ystride + P.ioff[p];
-      for (int i = tid; i < m; i += NTH) {
-        if (i < npiv) y[i] = t[i];
-        else U[i - npiv] = t[i];
-      }
-      __syncthreads();
-      return;
-    }
     for (int jb = 0; jb < npiv;) {
       const int je = block_end(ptype, jb, npiv, chol);
       const int bs = je - jb;
+      const int ldb = P.lds_b;
+      double* Lb = t + P.lds_t;  // bs x bs block, column major, leading dimension ldb
+      if (!REG) {
+        stage_block<T, NTH>(L, m, jb, bs, Lb, ldb);
+        __syncthreads();
+      }
       if (w == 0) {
-        // one wave, lane = row of the block: the row's block entries in
-        // registers (one round of loads from clamped addresses), then the
-        // column steps with readlane broadcasts
+        // one wave, lane = row of the block; one short step per column, the
+        // block's entries eight columns at a time (REG: the whole row in
+        // registers, one round of loads from HBM; else from LDS).  Column c
+        // updates the rows below it; the first column of a 2x2 pair skips
+        // the pair's second row (that slot is D's coupling, not L), which
+        // takes its own column's step next with its value unchanged.
         const int row = jb + lane;
         double tr = lane < bs ? t[row] : 0.0;
         const double dr = (chol && lane < bs) ? dv[row] : 1.0;
         const int pt = (!chol && lane < bs) ? ptype[row] : 1;
-        const size_t ro = (size_t)min(row, m - 1);
-        double lr[kSB];
+        auto steps8 = [&](int c0, const double(&lq)[8]) {
+          if (chol) {  // L scaled by the diagonal
 #pragma unroll
-        for (int c = 0; c < kSB; c++) lr[c] = (double)L[ro + (size_t)(jb + min(c, bs - 1)) * m];
+            for (int u = 0; u < 8; u++) {
+              const int c = c0 + u;
+              if (c >= bs) continue;
+              if (lane == c) tr = tr / dr;
+              const double tc = lane_d(tr, c);
+              if (lane > c && lane < bs) tr -= lq[u] * tc;
+            }
+          } else {
 #pragma unroll
-        for (int c = 0; c < kSB; c++) {
-          if (c >= bs) continue;
-          const int ty = __builtin_amdgcn_readlane(pt, c);
-          if (ty == 1) {
-            if (chol && lane == c) tr = tr / dr;
-            const double tc = lane_d(tr, c);
-            if (lane > c && lane < bs) tr -= lr[c] * tc;
-          } else if (ty == 2) {
-            const double ta = lane_d(tr, c), tb = lane_d(tr, c + 1);
-            if (lane > c + 1 && lane < bs) tr -= lr[c] * ta + lr[min(c + 1, kSB - 1)] * tb;
+            for (int u = 0; u < 8; u++) {
+              const int c = c0 + u;
+              if (c >= bs) continue;
+              const int ty = __builtin_amdgcn_readlane(pt, c);
+              const double tc = lane_d(tr, c);
+              if (lane > c + (ty == 2) && lane < bs) tr -= lq[u] * tc;
+            }
+          }
+        };
+        if constexpr (REG) {
+          const size_t ro = (size_t)min(row, m - 1);
+          double lr[kSB];
+#pragma unroll
+          for (int c = 0; c < kSB; c++) lr[c] = (double)L[ro + (size_t)(jb + min(c, bs - 1)) * m];
+#pragma unroll
+          for (int c0 = 0; c0 < kSB; c0 += 8) {
+            if (c0 >= bs) continue;
+            double lq[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) lq[u] = lr[c0 + u];
+            steps8(c0, lq);
+          }
+        } else {
+          const double* Lr = Lb + min(lane, bs - 1);
+          for (int c0 = 0; c0 < bs; c0 += 8) {
+            double lq[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) lq[u] = Lr[min(c0 + u, bs - 1) * ldb];
+            steps8(c0, lq);
           }
         }
         if (lane < bs) t[row] = tr;
@@ -912,17 +969,16 @@ __device__ __attribute__((always_inline)) inline void forward_node(
   }
 }
 
-template <class T, int NTH, bool BIG>
+template <class T, int NTH, bool REG>
 __global__ void __launch_bounds__(NTH) k_forward(DevPlan P, const int* nodes,
                                                  double* x, int ldx, int nrhs,
-                                                 long long ustride, int chol, double* Y,
-                                                 long long ystride) {
+                                                 long long ustride, int chol) {
   extern __shared__ double t[];
   __shared__ ChildView<T> s_cv[kChildCache];
   __shared__ long long s_uo[kChildCache];
   // one node per workgroup, one right-hand side per grid row
-  forward_node<T, NTH, BIG>(P, nodes[blockIdx.x], blockIdx.y, x, ldx, ustride, chol, Y, ystride,
-                            t, s_cv, s_uo);
+  forward_node<T, NTH, REG>(P, nodes[blockIdx.x], blockIdx.y, x, ldx, ustride, chol, t, s_cv,
+                            s_uo);
 }
 
 // A run of levels holding one small node each (a chain: each node's
@@ -936,18 +992,13 @@ __global__ void __launch_bounds__(NTH) k_forward_chain(DevPlan P, const int* nod
   __shared__ ChildView<T> s_cv[kChildCache];
   __shared__ long long s_uo[kChildCache];
   for (int q = 0; q < count; q++)
-    forward_node<T, NTH, false>(P, nodes[q], blockIdx.y, x, ldx, ustride, chol, nullptr, 0, t,
-                                s_cv, s_uo);
+    forward_node<T, NTH, true>(P, nodes[q], blockIdx.y, x, ldx, ustride, chol, t, s_cv, s_uo);
 }
 
 // Backward: t1 = x[pivots] - L21^T x[rows below], then the L11^T back solve
 // in blocks from the bottom (dot products over the rows below a block by
 // the whole workgroup, the diagonal block by one wave in registers).
 // BIG nodes are solved by k_bwd_gemv and the panel steps (k_bwd_step).
-// t[j] -= sum_{i0 <= i < i1} L[i, j] t[i] for the columns j0 <= j < j1:
-// one wave per column, lanes over i in steps of kWave (strided_dot's order),
-// then a butterfly.  With at most two terms per lane, CB columns of a wave
-// have their loads in flight together.
 template <class T, int NTH>
 __device__ __attribute__((always_inline)) inline void col_dots(const T* L, int m, double* t,
                                                                int j0, int j1, int i0, int i1) {
@@ -985,7 +1036,7 @@ __device__ __attribute__((always_inline)) inline void col_dots(const T* L, int m
   }
 }
 
-template <class T, int NTH>
+template <class T, int NTH, bool REG>
 __device__ __attribute__((always_inline)) inline void backward_node(const DevPlan& P, int p, int r,
                                                                     double* x, int ldx, int chol,
                                                                     double* t) {
@@ -1017,30 +1068,67 @@ __device__ __attribute__((always_inline)) inline void backward_node(const DevPla
         col_dots<T, NTH>(L, m, t, jb, je, je, npiv);
         __syncthreads();
       }
+      const int ldb = P.lds_b;
+      double* Lb = t + P.lds_t;
+      if (!REG) {
+        stage_block<T, NTH>(L, m, jb, bs, Lb, ldb);
+        __syncthreads();
+      }
       if (w == 0) {
-        // lane c owns column jb + c of the block: the column's block
-        // entries in registers (one round of loads from clamped addresses)
+        // lane c owns column jb + c of the block; one short step per row,
+        // from the bottom, the block's entries eight rows at a time (REG:
+        // the whole column in registers, one round of loads from HBM; else
+        // from LDS).  Row i is final once the rows after it are in acc (a
+        // 2x2 pair's two rows both at its second row, whose step then skips
+        // the pair's first column: that slot is D's coupling, not L); its
+        // value then enters acc.
         const int col = jb + lane;
         const double dc = (chol && lane < bs) ? dv[col] : 1.0;
         double tv = lane < bs ? t[col] : 0.0;
         double acc = 0.0;
         const int pt = (!chol && lane < bs) ? ptype[col] : 1;
-        const T* Lc = L + (size_t)min(col, je - 1) * m + jb;
-        double lc[kSB];
+        auto steps8 = [&](int i0, const double(&lq)[8]) {
+          if (chol) {
 #pragma unroll
-        for (int i = 0; i < kSB; i++) lc[i] = (double)Lc[min(i, bs - 1)];
+            for (int u = 0; u < 8; u++) {
+              const int i = i0 - u;
+              if (i < 0 || i >= bs) continue;
+              if (lane == i) tv = (tv - acc) / dc;
+              const double ti = lane_d(tv, i);
+              if (lane < i) acc += lq[u] * ti;
+            }
+          } else {
 #pragma unroll
-        for (int i = kSB - 1; i >= 0; i--) {
-          if (i >= bs) continue;
-          const int ty = __builtin_amdgcn_readlane(pt, i);
-          if (ty == 1) {
-            if (lane == i) tv = chol ? (tv - acc) / dc : tv - acc;
-            const double ti = lane_d(tv, i);
-            if (lane < i) acc += lc[i] * ti;
-          } else if (ty == 3) {
-            if (lane == i || lane == i - 1) tv -= acc;
-            const double ti = lane_d(tv, i), tj = lane_d(tv, max(i - 1, 0));
-            if (lane < i - 1) acc += lc[i] * ti + lc[max(i - 1, 0)] * tj;
+            for (int u = 0; u < 8; u++) {
+              const int i = i0 - u;
+              if (i < 0 || i >= bs) continue;
+              const int ty = __builtin_amdgcn_readlane(pt, i);
+              if ((ty != 2 && lane == i) || (ty == 3 && lane == i - 1)) tv = tv - acc;
+              const double ti = lane_d(tv, i);
+              if (lane < i - (ty == 3)) acc += lq[u] * ti;
+            }
+          }
+        };
+        if constexpr (REG) {
+          const T* Lc = L + (size_t)min(col, je - 1) * m + jb;
+          double lc[kSB];
+#pragma unroll
+          for (int i = 0; i < kSB; i++) lc[i] = (double)Lc[min(i, bs - 1)];
+#pragma unroll
+          for (int i0 = kSB - 1; i0 >= 0; i0 -= 8) {
+            if (i0 - 7 >= bs) continue;
+            double lq[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) lq[u] = lc[i0 - u];
+            steps8(i0, lq);
+          }
+        } else {
+          const double* Lc = Lb + (size_t)min(lane, bs - 1) * ldb;
+          for (int i0 = bs - 1; i0 >= 0; i0 -= 8) {
+            double lq[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) lq[u] = Lc[max(i0 - u, 0)];
+            steps8(i0, lq);
           }
         }
         if (lane < bs) t[col] = tv;
@@ -1053,12 +1141,12 @@ __device__ __attribute__((always_inline)) inline void backward_node(const DevPla
   }
 }
 
-template <class T, int NTH>
+template <class T, int NTH, bool REG>
 __global__ void __launch_bounds__(NTH) k_backward(DevPlan P, const int* nodes,
                                                   double* x, int ldx, int nrhs, int chol) {
   extern __shared__ double t[];
   // one node per workgroup, one right-hand side per grid row
-  backward_node<T, NTH>(P, nodes[blockIdx.x], blockIdx.y, x, ldx, chol, t);
+  backward_node<T, NTH, REG>(P, nodes[blockIdx.x], blockIdx.y, x, ldx, chol, t);
 }
 
 // The backward sweep of a chain (k_forward_chain): the nodes in reverse
@@ -1068,7 +1156,7 @@ __global__ void __launch_bounds__(NTH) k_backward_chain(DevPlan P, const int* no
                                                         double* x, int ldx, int chol) {
   extern __shared__ double t[];
   for (int q = count - 1; q >= 0; q--)
-    backward_node<T, NTH>(P, nodes[q], blockIdx.y, x, ldx, chol, t);
+    backward_node<T, NTH, true>(P, nodes[q], blockIdx.y, x, ldx, chol, t);
 }
 
 
@@ -1912,15 +2000,15 @@ struct Handle {
   // Y holds a large node's pivot vector
   std::vector<int> sv_ptr, sv_nodes, bg_ptr, bg_node, bg_tile;
   // sv nodes of a level: the first sv_tiny[l] have cap <= kTinyFront and are
-  // solved by one wave each; LDS per (level, group): sv_lt doubles
-  std::vector<int> sv_tiny, sv_lt;
+  // solved by one wave each; LDS shape per (level, group): sv_lt, sv_lb
+  std::vector<int> sv_tiny, sv_lt, sv_lb;
   // runs of levels holding one small node each and no large node (chains,
   // e.g. banded KKTs under AMD): the solve walks a run in one launch
   // (k_forward_chain, k_backward_chain).  sv_run_end[l] / sv_run_start[e]:
-  // the run [l, e] (-1 elsewhere), its LDS doubles sv_run_lt.
+  // the run [l, e] (-1 elsewhere), its LDS shape sv_run_lt / sv_run_lb.
   // KKT_LDL_SOLVE_CHAINS=0 keeps one launch per level (same results, bit
   // for bit).
-  std::vector<int> sv_run_end, sv_run_start, sv_run_lt;
+  std::vector<int> sv_run_end, sv_run_start, sv_run_lt, sv_run_lb;
   bool solve_chains = true;
   std::vector<int> fs_lvl, fs_ptr, fs_node, fs_tile, bs_lvl, bs_ptr, bs_node, bs_tile;
   DBuf<int> d_sv_nodes, d_bg_node, d_bg_tile, d_fs_node, d_fs_tile, d_bs_node, d_bs_tile;
@@ -2263,6 +2351,7 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   H.sv_ptr.assign(nlev + 1, 0);
   H.sv_tiny.assign(nlev, 0);
   H.sv_lt.assign(2 * (size_t)nlev, 0);
+  H.sv_lb.assign(2 * (size_t)nlev, 0);
   H.bg_ptr.assign(nlev + 1, 0);
   H.sv_nodes.clear();
   H.bg_node.clear();
@@ -2311,13 +2400,15 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     H.sv_ptr[l + 1] = (int)H.sv_nodes.size();
     H.bg_ptr[l + 1] = (int)H.bg_node.size();
     // tiny fronts (cap <= kTinyFront) lead the level's list (LDS fronts are
-    // sorted by cap); each group gets LDS for its largest front vector
+    // sorted by cap); each group gets LDS for its largest front vector and
+    // its largest diagonal block
     for (int q = H.sv_ptr[l]; q < H.sv_ptr[l + 1]; q++) {
       const int k = H.sv_nodes[q];
       const bool tiny = H.cap[k] <= kTinyFront && q - H.sv_ptr[l] == H.sv_tiny[l];
       if (tiny) H.sv_tiny[l]++;
       const int g = 2 * l + (tiny ? 0 : 1);
       H.sv_lt[g] = std::max(H.sv_lt[g], (H.cap[k] + 7) & ~7);
+      H.sv_lb[g] = std::max(H.sv_lb[g], std::min(kSB, H.cap[k] - H.cs[k]));
     }
     // forward step t: rows [b1, m) with b1 > t*kPS; tile 0 always exists
     H.fs_lvl[l] = (int)H.fs_ptr.size() - 1;
@@ -2354,6 +2445,7 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
   H.sv_run_end.assign(nlev, -1);
   H.sv_run_start.assign(nlev, -1);
   H.sv_run_lt.assign(nlev, 0);
+  H.sv_run_lb.assign(nlev, 0);
   auto single = [&](int l) {
     return H.sv_ptr[l + 1] - H.sv_ptr[l] == 1 && H.large_ptr[l + 1] == H.large_ptr[l];
   };
@@ -2367,10 +2459,14 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
     if (e > l) {
       H.sv_run_end[l] = e;
       H.sv_run_start[e] = l;
-      int lt = 0;
-      for (int q = H.sv_ptr[l]; q < H.sv_ptr[e + 1]; q++)
-        lt = std::max(lt, (H.cap[H.sv_nodes[q]] + 7) & ~7);
+      int lt = 0, lb = 1;
+      for (int q = H.sv_ptr[l]; q < H.sv_ptr[e + 1]; q++) {
+        const int k = H.sv_nodes[q];
+        lt = std::max(lt, (H.cap[k] + 7) & ~7);
+        lb = std::max(lb, std::min(kSB, H.cap[k] - H.cs[k]));
+      }
       H.sv_run_lt[l] = lt;
+      H.sv_run_lb[l] = lb;
     }
     l = e + 1;
   }
@@ -2932,11 +3028,13 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   const int chol = H.opt.solver_type == KKT_LDL_CHOLESKY;
   const int lds_t = (H.max_cap + 31) & ~31;
   P.lds_t = lds_t;
+  P.lds_b = kSB;
   // per (level, group) plan copies with their LDS shape
   auto sv_plan = [&](int g, size_t* bytes) {
     DevPlan Q = P;
     Q.lds_t = H.sv_lt[g];
-    *bytes = sizeof(double) * (size_t)Q.lds_t;
+    Q.lds_b = std::max(1, H.sv_lb[g]);
+    *bytes = sizeof(double) * ((size_t)Q.lds_t + (size_t)Q.lds_b * Q.lds_b);
     if (*bytes > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
     return Q;
   };
@@ -2949,7 +3047,8 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   auto chain_plan = [&](int l0, size_t* bytes) {
     DevPlan Q = P;
     Q.lds_t = H.sv_run_lt[l0];
-    *bytes = sizeof(double) * (size_t)Q.lds_t;
+    Q.lds_b = H.sv_run_lb[l0];
+    *bytes = sizeof(double) * ((size_t)Q.lds_t + (size_t)Q.lds_b * Q.lds_b);
     if (*bytes > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
     return Q;
   };
@@ -2965,15 +3064,26 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
       continue;
     }
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
+    // launches of few workgroups are latency bound: their diagonal blocks
+    // are read straight into registers (REG); larger ones stage them in LDS
+    // and keep more workgroups per CU (same results either way)
     if (sm > s0) {
       const DevPlan Q = sv_plan(2 * l, &lds);
-      hipLaunchKernelGGL((k_forward<T, kWave, false>), dim3(sm - s0, nrhs), dim3(kWave), lds, st, Q,
-                         H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol, Y, ys);
+      if ((sm - s0) * nrhs <= kRegLaunch)
+        hipLaunchKernelGGL((k_forward<T, kWave, true>), dim3(sm - s0, nrhs), dim3(kWave), lds, st,
+                           Q, H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol);
+      else
+        hipLaunchKernelGGL((k_forward<T, kWave, false>), dim3(sm - s0, nrhs), dim3(kWave), lds, st,
+                           Q, H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol);
     }
     if (s1 > sm) {
       const DevPlan Q = sv_plan(2 * l + 1, &lds);
-      hipLaunchKernelGGL((k_forward<T, kTPB, false>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st, Q,
-                         H.d_sv_nodes.p + sm, dx, ldx, nrhs, us, chol, Y, ys);
+      if ((s1 - sm) * nrhs <= kRegLaunch)
+        hipLaunchKernelGGL((k_forward<T, kTPB, true>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
+                           Q, H.d_sv_nodes.p + sm, dx, ldx, nrhs, us, chol);
+      else
+        hipLaunchKernelGGL((k_forward<T, kTPB, false>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
+                           Q, H.d_sv_nodes.p + sm, dx, ldx, nrhs, us, chol);
     }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
@@ -3039,13 +3149,21 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
     if (sm > s0) {
       const DevPlan Q = sv_plan(2 * l, &lds);
-      hipLaunchKernelGGL((k_backward<T, kWave>), dim3(sm - s0, nrhs), dim3(kWave), lds, st, Q,
-                         H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol);
+      if ((sm - s0) * nrhs <= kRegLaunch)
+        hipLaunchKernelGGL((k_backward<T, kWave, true>), dim3(sm - s0, nrhs), dim3(kWave), lds, st,
+                           Q, H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol);
+      else
+        hipLaunchKernelGGL((k_backward<T, kWave, false>), dim3(sm - s0, nrhs), dim3(kWave), lds, st,
+                           Q, H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol);
     }
     if (s1 > sm) {
       const DevPlan Q = sv_plan(2 * l + 1, &lds);
-      hipLaunchKernelGGL((k_backward<T, kTPB>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st, Q,
-                         H.d_sv_nodes.p + sm, dx, ldx, nrhs, chol);
+      if ((s1 - sm) * nrhs <= kRegLaunch)
+        hipLaunchKernelGGL((k_backward<T, kTPB, true>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
+                           Q, H.d_sv_nodes.p + sm, dx, ldx, nrhs, chol);
+      else
+        hipLaunchKernelGGL((k_backward<T, kTPB, false>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
+                           Q, H.d_sv_nodes.p + sm, dx, ldx, nrhs, chol);
     }
   }
   HIPCHECK(hipGetLastError());
