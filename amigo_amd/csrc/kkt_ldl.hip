@@ -1344,6 +1344,50 @@ __global__ void __launch_bounds__(kWave) k_panel_inv2(DevPlan P, const int* inod
   for (int c = 0; c < kPB; c++) X[lane + c * kPB] = S[lane * LB + c];
 }
 
+// Entry i (post-pivot order) of a BIG node's front vector for the
+// right-hand sides r0 .. r0 + nb - 1 (RB slots, the last repeated): x of the
+// row's variable (zero for delayed and contribution rows) plus the
+// children's update-vector entries in child order, four children's lookups
+// and loads in flight at a time (the order of k_forward's gather).
+template <class T, int RB>
+__device__ __attribute__((always_inline)) inline void gather_big_row(
+    const DevPlan& P, int i, int fs, int ns, int din, const int* pre, int vp, int c0, int nch,
+    const ChildView<T>* s_cv, const long long* s_uo, const double* x, int ldx, int r0, int nb,
+    long long ustride, double (&v)[RB]) {
+  const int pi = i < fs ? pre[i] : i;
+  const int var = P.node_to_var[vp + min(pi, ns - 1)];
+#pragma unroll
+  for (int rr = 0; rr < RB; rr++) {
+    const double xv = x[(size_t)(r0 + min(rr, nb - 1)) * ldx + var];
+    v[rr] = pi < ns ? xv : 0.0;
+  }
+  for (int q0 = 0; q0 < nch; q0 += 4) {
+    int rc[4];
+    ChildView<T> cvs[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int qq = min(q0 + u, nch - 1);
+      cvs[u] = qq < kChildCache ? s_cv[qq] : load_child<T>(P, P.ch[c0 + qq]);
+      const int rq = child_row_nb(P, cvs[u], pi, ns, din);
+      rc[u] = q0 + u < nch ? rq : -1;
+    }
+#pragma unroll
+    for (int rr = 0; rr < RB; rr++) {
+      double uv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int qq = min(q0 + u, nch - 1);
+        const long long uo = qq < kChildCache ? s_uo[qq] : P.uoff[P.ch[c0 + qq]];
+        uv[u] = P.U[uo + (long long)(r0 + min(rr, nb - 1)) * ustride +
+                    max(rc[u] - cvs[u].npiv, 0)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (rc[u] >= 0) v[rr] += uv[u];
+    }
+  }
+}
+
 // Forward step s of the large nodes of a level: the 128 columns
 // [b0, b0 + 128) as two 64-blocks a, b.  Y holds the pivot rows of each
 // front vector (gathered by k_forward and updated by the earlier steps), U
@@ -1355,7 +1399,11 @@ __global__ void __launch_bounds__(kWave) k_panel_inv2(DevPlan P, const int* inod
 // the same step: the next step's block is unit lower triangular, so the
 // solved entry of row b1 is its y after this step's update, which tile 0
 // computes.
-template <class T, int RB>
+// G (step 0 only): the workgroup gathers the front vector entries it needs
+// itself (the step's rows y_s and its own tile's rows, as k_gather_big would
+// write them), instead of a gather launch before the step; its rows are
+// written once, after the update.
+template <class T, int RB, bool G>
 __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
                                                    const int* wtile, int s, double* x,
                                                    int ldx, int nrhs, long long ustride,
@@ -1366,6 +1414,9 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   __shared__ double ys[RB][kPS];
   __shared__ double xp[RB][kPS];
   __shared__ double part[2][RB][NW][kWave];
+  __shared__ double gown[G ? RB : 1][kWave];
+  __shared__ ChildView<T> s_cv[G ? kChildCache : 1];
+  __shared__ long long s_uo[G ? kChildCache : 1];
   const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
   // everything per node depends on p alone: one round of loads
   const int m = P.m[p], npiv = P.npiv[p];
@@ -1412,9 +1463,44 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   // first (each sum keeps the order of the one-right-hand-side form)
   const int r0 = blockIdx.y * RB;
   const int nr = min(nrhs - r0, RB);
-  for (int e = tid; e < RB * kPS; e += kTPB) {
-    const int rr = e / kPS, c = e % kPS;
-    ys[rr][c] = (rr < nr && c < w) ? Y[(long long)(r0 + rr) * ystride + io + b0 + c] : 0.0;
+  if constexpr (G) {
+    // the node's children, then the lanes of wave 0 gather the tile's rows
+    // and the next kPS threads the step's rows y_s
+    const int ns = P.ns[p], din = P.din[p], fs = ns + din;
+    const int* pre = P.pre + io;
+    const int c0 = P.chptr[p], nch = P.chptr[p + 1] - c0;
+    const int vp = P.vptr[p];
+    for (int q = tid; q < min(nch, kChildCache); q += kTPB) {
+      s_cv[q] = load_child<T>(P, P.ch[c0 + q]);
+      s_uo[q] = P.uoff[P.ch[c0 + q]];
+    }
+    __syncthreads();
+    if (tid < kWave) {
+      if (i < m) {
+        double v[RB];
+        gather_big_row<T, RB>(P, i, fs, ns, din, pre, vp, c0, nch, s_cv, s_uo, x, ldx, r0, nr,
+                              ustride, v);
+#pragma unroll
+        for (int rr = 0; rr < RB; rr++) gown[rr][tid] = v[rr];
+      }
+    } else if (tid - kWave < w) {
+      const int c = tid - kWave;
+      double v[RB];
+      gather_big_row<T, RB>(P, b0 + c, fs, ns, din, pre, vp, c0, nch, s_cv, s_uo, x, ldx, r0,
+                            nr, ustride, v);
+#pragma unroll
+      for (int rr = 0; rr < RB; rr++)
+        if (rr < nr) ys[rr][c] = v[rr];
+    }
+    for (int e = tid; e < RB * kPS; e += kTPB) {
+      const int rr = e / kPS, c = e % kPS;
+      if (rr >= nr || c >= w) ys[rr][c] = 0.0;
+    }
+  } else {
+    for (int e = tid; e < RB * kPS; e += kTPB) {
+      const int rr = e / kPS, c = e % kPS;
+      ys[rr][c] = (rr < nr && c < w) ? Y[(long long)(r0 + rr) * ystride + io + b0 + c] : 0.0;
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -1458,39 +1544,46 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
       const double sum = ((part[0][rr][0][lane] + part[0][rr][1][lane]) + part[0][rr][2][lane]) +
                          part[0][rr][3][lane];
       if (i < npiv) {
-        const double yn = y[i] - sum;
+        const double yn = (G ? gown[rr][lane] : y[i]) - sum;
         y[i] = yn;
         if (cut && tile == 0 && lane == 0) {  // i == b1: the pair (b1 - 1, b1)
           const int j = b1 - 1;
           const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
           const double dinv = 1.0 / (d11 * d22 - d21 * d21);
           const double ta = xp[rr][w - 1], tb = yn;
-          xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
+          // (G: the step's own pivot results go to Y, see below)
+          (G ? y[j] : xr[unpiv(vars[j])]) = dinv * (d22 * ta - d21 * tb);
           xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
         }
       } else {
-        P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)] -= sum;
+        double& u = P.U[P.uoff[p] + (long long)r * ustride + (i - npiv)];
+        u = (G ? gown[rr][lane] : u) - sum;
       }
     }
   }
   if (tile == 0) {
     // 1x1 pivots and 2x2 pivots inside the step (also across its 64-column
     // blocks); a pair split by the step boundary is written above (second
-    // pivot first in the step: by the step before)
+    // pivot first in the step: by the step before).  G: the other
+    // workgroups of the node read x of these pivots' variables (their
+    // right-hand side) while gathering, so the results go to Y (rows
+    // 0 .. b1 - 1, which nothing else reads) and the backward sweep takes
+    // them from there.
     for (int e = tid; e < nr * kPS; e += kTPB) {
       const int rr = e / kPS, jj = e % kPS;
       if (jj >= w) continue;
       double* xr = x + (size_t)(r0 + rr) * ldx;
+      double* yr = Y + (long long)(r0 + rr) * ystride + io;
       const int j = b0 + jj;
       const int ty = ptype[j];
       if (ty == 1) {
-        xr[unpiv(vars[j])] = xp[rr][jj] / dv[j];
+        (G ? yr[j] : xr[unpiv(vars[j])]) = xp[rr][jj] / dv[j];
       } else if (ty == 2 && jj + 1 < w) {
         const double d11 = dv[j], d21 = dv2[j], d22 = dv[j + 1];
         const double dinv = 1.0 / (d11 * d22 - d21 * d21);
         const double ta = xp[rr][jj], tb = xp[rr][jj + 1];
-        xr[unpiv(vars[j])] = dinv * (d22 * ta - d21 * tb);
-        xr[unpiv(vars[j + 1])] = dinv * (d11 * tb - d21 * ta);
+        (G ? yr[j] : xr[unpiv(vars[j])]) = dinv * (d22 * ta - d21 * tb);
+        (G ? yr[j + 1] : xr[unpiv(vars[j + 1])]) = dinv * (d11 * tb - d21 * ta);
       }
     }
   }
@@ -1619,11 +1712,14 @@ __global__ void __launch_bounds__(kTPB) k_bwd_step(DevPlan P, const int* wnode,
 constexpr int kGemvCols = 16;
 // RB right-hand sides per workgroup with their x rows staged in LDS; RB = 0:
 // one right-hand side, x read directly (rows below the pivots beyond LDS).
+// yfw: the level's forward sweep left the results of each node's first
+// kPS pivots in Y (the first step gathered its own front vector), so they
+// are read there instead of from x.
 template <class T, int RB>
 __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
                                                    const int* wtile, const double* x,
                                                    int ldx, int nrhs, double* Y,
-                                                   long long ystride, int ldxs) {
+                                                   long long ystride, int ldxs, int yfw) {
   extern __shared__ double xs[];  // x of the rows below the pivots, RB right-hand sides
   const int p = wnode[blockIdx.x];
   const int m = P.m[p], npiv = P.npiv[p];
@@ -1635,6 +1731,7 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const T* L = front_ptr<T>(P, p);
   const int nr = m - npiv;
+  const int yk = yfw ? min(npiv, kPS) : 0;
   constexpr int RR = RB > 0 ? RB : 1;
   const int r0 = blockIdx.y * RR;
   const int nb = min(nrhs - r0, RR);
@@ -1715,7 +1812,8 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
     const int rr = lane / CW, j = jb + lane % CW;
     if (rr < nb && j < npiv) {
       const double* xr = x + (size_t)(r0 + rr) * ldx;
-      Y[(long long)(r0 + rr) * ystride + P.ioff[p] + j] = xr[unpiv(vars[j])] - v;
+      double* yj = Y + (long long)(r0 + rr) * ystride + P.ioff[p] + j;
+      *yj = (j < yk ? *yj : xr[unpiv(vars[j])]) - v;
     }
   }
 }
@@ -1751,39 +1849,9 @@ __global__ void __launch_bounds__(1024) k_gather_big(DevPlan P, const int* nodes
   const int nb = min(nrhs - r0, RB);
   double* U = P.U + P.uoff[p];
   for (int i = tid; i < m; i += NTH) {
-    const int pi = i < fs ? pre[i] : i;
-    const int var = P.node_to_var[vp + min(pi, ns - 1)];
     double v[RB];
-#pragma unroll
-    for (int rr = 0; rr < RB; rr++) {
-      const double xv = x[(size_t)(r0 + min(rr, nb - 1)) * ldx + var];
-      v[rr] = pi < ns ? xv : 0.0;
-    }
-    for (int q0 = 0; q0 < nch; q0 += 4) {
-      int rc[4];
-      ChildView<T> cvs[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int qq = min(q0 + u, nch - 1);
-        cvs[u] = qq < kChildCache ? s_cv[qq] : load_child<T>(P, P.ch[c0 + qq]);
-        const int rq = child_row_nb(P, cvs[u], pi, ns, din);
-        rc[u] = q0 + u < nch ? rq : -1;
-      }
-#pragma unroll
-      for (int rr = 0; rr < RB; rr++) {
-        double uv[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int qq = min(q0 + u, nch - 1);
-          const long long uo = qq < kChildCache ? s_uo[qq] : P.uoff[P.ch[c0 + qq]];
-          uv[u] = P.U[uo + (long long)(r0 + min(rr, nb - 1)) * ustride +
-                      max(rc[u] - cvs[u].npiv, 0)];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (rc[u] >= 0) v[rr] += uv[u];
-      }
-    }
+    gather_big_row<T, RB>(P, i, fs, ns, din, pre, vp, c0, nch, s_cv, s_uo, x, ldx, r0, nb,
+                          ustride, v);
 #pragma unroll
     for (int rr = 0; rr < RB; rr++) {
       if (rr >= nb) break;
@@ -2018,6 +2086,12 @@ struct Handle {
   std::vector<int> inv_node, inv_panel, inv2_node, inv2_pair;
   long long pinv_total = 0;
   std::vector<int> lvl_steps;  // per level: panel steps needed by the last factorization
+  // per level: every large node has pivots (last factorization), so the
+  // forward gather is done by the first step's workgroups (k_fwd_step<...,
+  // true>) instead of its own launch; KKT_LDL_FUSE_GATHER=0 keeps the launch
+  // (same results, bit for bit)
+  std::vector<char> lvl_gfuse;
+  bool fuse_gather = true;
   DBuf<long long> d_pinvoff;
   DBuf<int> d_inv_node, d_inv_panel, d_inv2_node, d_inv2_pair;
   DBuf<double> d_pinv;
@@ -2991,9 +3065,13 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
   // the solve launches only the panel steps that hold pivots
   const int nlev = (int)H.lvl_ptr.size() - 1;
   H.lvl_steps.assign(nlev, 0);
+  H.lvl_gfuse.assign(nlev, 1);
   for (int l = 0; l < nlev; l++)
-    for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++)
-      H.lvl_steps[l] = std::max(H.lvl_steps[l], (H.h_npiv[H.large_nodes[q]] + kPS - 1) / kPS);
+    for (int q = H.large_ptr[l]; q < H.large_ptr[l + 1]; q++) {
+      const int np = H.h_npiv[H.large_nodes[q]];
+      H.lvl_steps[l] = std::max(H.lvl_steps[l], (np + kPS - 1) / kPS);
+      if (np == 0) H.lvl_gfuse[l] = 0;
+    }
   int nd = 0, n2 = 0, nz = 0, nf = 0;
   long long stored = 0;
   for (int k = 0; k < H.N; k++) {
@@ -3043,6 +3121,12 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   const long long ys = H.i_total, us = H.u_total;
   double* Y = H.d_Y.p;
   constexpr int kGatherThreads = 1024;
+  // levels whose forward gather is done by the first step (see k_fwd_step);
+  // one right-hand side only (with more, every step workgroup would gather
+  // its rows for each group of right-hand sides again)
+  auto gfuse = [&](int l) {
+    return H.fuse_gather && nrhs == 1 && (int)H.lvl_gfuse.size() > l && H.lvl_gfuse[l] != 0;
+  };
   // a chain's LDS shape and launch (one workgroup per right-hand side)
   auto chain_plan = [&](int l0, size_t* bytes) {
     DevPlan Q = P;
@@ -3087,25 +3171,41 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
-      if (nrhs == 1)
-        hipLaunchKernelGGL((k_gather_big<T, 1>), dim3(g1 - g0, 1), dim3(kGatherThreads), 0, st, P,
-                           H.d_large_nodes.p + g0, dx, ldx, nrhs, us, Y, ys);
-      else
-        hipLaunchKernelGGL((k_gather_big<T, kGatherRhs>),
-                           dim3(g1 - g0, (nrhs + kGatherRhs - 1) / kGatherRhs), dim3(kGatherThreads),
-                           0, st, P, H.d_large_nodes.p + g0, dx, ldx, nrhs, us, Y, ys);
+      // the gather done by the first step's workgroups where every large
+      // node of the level has pivots (else a gather launch)
+      const bool gf = gfuse(l);
+      if (!gf) {
+        if (nrhs == 1)
+          hipLaunchKernelGGL((k_gather_big<T, 1>), dim3(g1 - g0, 1), dim3(kGatherThreads), 0, st,
+                             P, H.d_large_nodes.p + g0, dx, ldx, nrhs, us, Y, ys);
+        else
+          hipLaunchKernelGGL((k_gather_big<T, kGatherRhs>),
+                             dim3(g1 - g0, (nrhs + kGatherRhs - 1) / kGatherRhs),
+                             dim3(kGatherThreads), 0, st, P, H.d_large_nodes.p + g0, dx, ldx, nrhs,
+                             us, Y, ys);
+      }
       const int nst = std::min(H.fs_lvl[l + 1] - H.fs_lvl[l], H.lvl_steps[l]);
       for (int t = 0; t < nst; t++) {
         const int slot = H.fs_lvl[l] + t;
         const int f0 = H.fs_ptr[slot], f1 = H.fs_ptr[slot + 1];
-        if (nrhs == 1)
-          hipLaunchKernelGGL((k_fwd_step<T, 1>), dim3(f1 - f0, 1), dim3(kTPB), 0, st, P,
-                             H.d_fs_node.p + f0, H.d_fs_tile.p + f0, t, dx, ldx, nrhs, us, Y, ys);
-        else
-          hipLaunchKernelGGL((k_fwd_step<T, kStepRhs>),
-                             dim3(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs), dim3(kTPB), 0, st,
-                             P, H.d_fs_node.p + f0, H.d_fs_tile.p + f0, t, dx, ldx, nrhs, us, Y,
-                             ys);
+        const dim3 g1r(f1 - f0, 1), g4r(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs);
+        const int* fn = H.d_fs_node.p + f0;
+        const int* ft = H.d_fs_tile.p + f0;
+        if (gf && t == 0) {
+          if (nrhs == 1)
+            hipLaunchKernelGGL((k_fwd_step<T, 1, true>), g1r, dim3(kTPB), 0, st, P, fn, ft, t, dx,
+                               ldx, nrhs, us, Y, ys);
+          else
+            hipLaunchKernelGGL((k_fwd_step<T, kStepRhs, true>), g4r, dim3(kTPB), 0, st, P, fn, ft,
+                               t, dx, ldx, nrhs, us, Y, ys);
+        } else {
+          if (nrhs == 1)
+            hipLaunchKernelGGL((k_fwd_step<T, 1, false>), g1r, dim3(kTPB), 0, st, P, fn, ft, t,
+                               dx, ldx, nrhs, us, Y, ys);
+          else
+            hipLaunchKernelGGL((k_fwd_step<T, kStepRhs, false>), g4r, dim3(kTPB), 0, st, P, fn,
+                               ft, t, dx, ldx, nrhs, us, Y, ys);
+        }
       }
     }
   }
@@ -3123,16 +3223,17 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
       const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
+      const int yfw = gfuse(l) ? 1 : 0;
       // one right-hand side per workgroup (batching them here serialises a
       // workgroup's dot products and was measured slower); the x rows below
       // the pivots staged in LDS where they fit
       if (lds_bg <= 128 * 1024)
         hipLaunchKernelGGL((k_bwd_gemv<T, 1>), dim3(b1 - b0, nrhs), dim3(kTPB), lds_bg, st, P,
                            H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys,
-                           H.max_below);
+                           H.max_below, yfw);
       else
         hipLaunchKernelGGL((k_bwd_gemv<T, 0>), dim3(b1 - b0, nrhs), dim3(kTPB), 0, st, P,
-                           H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys, 0);
+                           H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys, 0, yfw);
       const int nst = std::min(H.bs_lvl[l + 1] - H.bs_lvl[l], H.lvl_steps[l]);
       for (int t = 0; t < nst; t++) {
         const int slot = H.bs_lvl[l] + t;
@@ -3345,6 +3446,7 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   if (const char* e = std::getenv("KKT_LDL_RL_FRONTS")) H.rl_fronts &= std::atoi(e) != 0;
   H.lds_front = std::min(H.lds_front, rl_max_front());
   if (const char* e = std::getenv("KKT_LDL_ROOT_BK")) H.root_bk = std::atoi(e) != 0;
+  if (const char* e = std::getenv("KKT_LDL_FUSE_GATHER")) H.fuse_gather = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_SOLVE_CHAINS")) H.solve_chains = std::atoi(e) != 0;
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));

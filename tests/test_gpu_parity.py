@@ -222,3 +222,43 @@ def test_multi_rhs_columns_bitwise_equal_single_solves(prec):
         xj = B[:, j].clone()
         s.solve(xj)
         assert torch.equal(xj, X[:, j]), j
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["poisson2d", "cartpole"])
+def test_first_step_gather_bitwise_equal_gather_launch(name, prec):
+    """Levels whose large nodes all hold pivots gather the front vector in
+    the first forward step's workgroups (each the step's rows and its own
+    tile's rows, with k_gather_big's sums) instead of a gather launch before
+    the step (KKT_LDL_FUSE_GATHER=0 when the handle is created; solves of
+    one right-hand side): the same solutions bit for bit, for 1 and 5
+    right-hand sides and the refined solve, under nested dissection."""
+    import os
+
+    import torch
+    from amigo_amd.ldl import FP32_REFINE
+    P = W.GENERATORS[name]()
+    hs = []
+    for flag in ("1", "0"):
+        os.environ["KKT_LDL_FUSE_GATHER"] = flag
+        try:
+            s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
+                          order=OrderingType.NESTED_DISSECTION,
+                          precision=FP32_REFINE if prec == "fp32" else 0)
+        finally:
+            del os.environ["KKT_LDL_FUSE_GATHER"]
+        assert s.factor(P.diag) == 0
+        hs.append(s)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    B = torch.randn(5, P.n, dtype=torch.float64, device="cuda", generator=g).t()
+    for nrhs in (1, 5):
+        xa, xb = B[:, :nrhs].clone(), B[:, :nrhs].clone()
+        hs[0].solve(xa)
+        hs[1].solve(xb)
+        assert torch.equal(xa, xb), nrhs
+    xa = torch.as_tensor(P.rhs, device="cuda").clone()
+    xb = xa.clone()
+    assert hs[0].solve(xa, refine=True) == 0
+    assert hs[1].solve(xb, refine=True) == 0
+    assert torch.equal(xa, xb)
+    assert _relres(P.matrix(), xa.cpu().numpy(), P.rhs) <= 1e-10
