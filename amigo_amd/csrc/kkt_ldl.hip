@@ -769,6 +769,7 @@ constexpr int kSB = 64;
 // register-resident diagonal block (REG)
 constexpr int kRegLaunch = 256;
 constexpr int kTinyFront = 64;  // fronts up to this dimension: one wave each
+constexpr int kTinyJoin = 64;   // at most this many tiny fronts join a level's larger ones
 
 // End of the solve block starting at jb: kSB columns, shortened so that a
 // 2x2 pivot pair is never split.
@@ -3108,10 +3109,11 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   P.lds_t = lds_t;
   P.lds_b = kSB;
   // per (level, group) plan copies with their LDS shape
-  auto sv_plan = [&](int g, size_t* bytes) {
+  // (g = 2 l + 1 with tiny fronts joining: the shape covers both groups)
+  auto sv_plan = [&](int g, size_t* bytes, bool join = false) {
     DevPlan Q = P;
-    Q.lds_t = H.sv_lt[g];
-    Q.lds_b = std::max(1, H.sv_lb[g]);
+    Q.lds_t = join ? std::max(H.sv_lt[g], H.sv_lt[g - 1]) : H.sv_lt[g];
+    Q.lds_b = std::max(1, join ? std::max(H.sv_lb[g], H.sv_lb[g - 1]) : H.sv_lb[g]);
     *bytes = sizeof(double) * ((size_t)Q.lds_t + (size_t)Q.lds_b * Q.lds_b);
     if (*bytes > 160 * 1024) throw Error(-6, "front too large for the LDS solve kernels");
     return Q;
@@ -3121,6 +3123,13 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   const long long ys = H.i_total, us = H.u_total;
   double* Y = H.d_Y.p;
   constexpr int kGatherThreads = 1024;
+  // the level's tiny fronts (one wave each) take their own launch, unless
+  // they are few and the level has larger small fronts: then they join
+  // those fronts' launch (same results)
+  auto tiny_alone = [&](int l) {
+    const int nt = H.sv_tiny[l], nall = H.sv_ptr[l + 1] - H.sv_ptr[l];
+    return !(nt <= kTinyJoin && nall > nt);
+  };
   // levels whose forward gather is done by the first step (see k_fwd_step);
   // one right-hand side only (with more, every step workgroup would gather
   // its rows for each group of right-hand sides again)
@@ -3147,7 +3156,9 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
       l = e;
       continue;
     }
-    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
+    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
+    // a few tiny fronts join the level's other small fronts in one launch
+    const int sm = s0 + (tiny_alone(l) ? H.sv_tiny[l] : 0);
     // launches of few workgroups are latency bound: their diagonal blocks
     // are read straight into registers (REG); larger ones stage them in LDS
     // and keep more workgroups per CU (same results either way)
@@ -3161,7 +3172,7 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
                            Q, H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol);
     }
     if (s1 > sm) {
-      const DevPlan Q = sv_plan(2 * l + 1, &lds);
+      const DevPlan Q = sv_plan(2 * l + 1, &lds, sm == s0 && H.sv_tiny[l] > 0);
       if ((s1 - sm) * nrhs <= kRegLaunch)
         hipLaunchKernelGGL((k_forward<T, kTPB, true>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
                            Q, H.d_sv_nodes.p + sm, dx, ldx, nrhs, us, chol);
@@ -3247,7 +3258,8 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
                              P, H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
       }
     }
-    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1], sm = s0 + H.sv_tiny[l];
+    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
+    const int sm = s0 + (tiny_alone(l) ? H.sv_tiny[l] : 0);
     if (sm > s0) {
       const DevPlan Q = sv_plan(2 * l, &lds);
       if ((sm - s0) * nrhs <= kRegLaunch)
@@ -3258,7 +3270,7 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
                            Q, H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol);
     }
     if (s1 > sm) {
-      const DevPlan Q = sv_plan(2 * l + 1, &lds);
+      const DevPlan Q = sv_plan(2 * l + 1, &lds, sm == s0 && H.sv_tiny[l] > 0);
       if ((s1 - sm) * nrhs <= kRegLaunch)
         hipLaunchKernelGGL((k_backward<T, kTPB, true>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
                            Q, H.d_sv_nodes.p + sm, dx, ldx, nrhs, chol);
