@@ -770,6 +770,11 @@ constexpr int kSB = 64;
 constexpr int kRegLaunch = 256;
 constexpr int kTinyFront = 64;  // fronts up to this dimension: one wave each
 constexpr int kTinyJoin = 64;   // at most this many tiny fronts join a level's larger ones
+// a level's small fronts share a launch with its large nodes' first launch
+// (k_fwd_level / k_bwd_level) when they are at most this many and the rows
+// staged by the backward products fit in kMergeLds bytes
+constexpr int kMergeSmall = 512;
+constexpr size_t kMergeLds = 64 * 1024;
 
 // End of the solve block starting at jb: kSB columns, shortened so that a
 // 2x2 pivot pair is never split.
@@ -1405,10 +1410,9 @@ __device__ __attribute__((always_inline)) inline void gather_big_row(
 // write them), instead of a gather launch before the step; its rows are
 // written once, after the update.
 template <class T, int RB, bool G>
-__global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
-                                                   const int* wtile, int s, double* x,
-                                                   int ldx, int nrhs, long long ustride,
-                                                   double* Y, long long ystride) {
+__device__ __attribute__((always_inline)) inline void fwd_step_wg(
+    const DevPlan& P, int bx, int by, const int* wnode, const int* wtile, int s, double* x,
+    int ldx, int nrhs, long long ustride, double* Y, long long ystride) {
   constexpr int NW = kTPB / kWave;
   constexpr int CQ = kPB / NW;  // block columns per wave (inverse products)
   constexpr int CR = kPS / NW;  // step columns per wave (row update)
@@ -1418,7 +1422,7 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   __shared__ double gown[G ? RB : 1][kWave];
   __shared__ ChildView<T> s_cv[G ? kChildCache : 1];
   __shared__ long long s_uo[G ? kChildCache : 1];
-  const int p = wnode[blockIdx.x], tile = wtile[blockIdx.x];
+  const int p = wnode[bx], tile = wtile[bx];
   // everything per node depends on p alone: one round of loads
   const int m = P.m[p], npiv = P.npiv[p];
   const T* L = front_ptr<T>(P, p);
@@ -1462,7 +1466,7 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
   // RB right-hand sides per grid row, on the operands loaded once, all
   // with one set of barriers: y_s of every right-hand side staged in LDS
   // first (each sum keeps the order of the one-right-hand-side form)
-  const int r0 = blockIdx.y * RB;
+  const int r0 = by * RB;
   const int nr = min(nrhs - r0, RB);
   if constexpr (G) {
     // the node's children, then the lanes of wave 0 gather the tile's rows
@@ -1588,6 +1592,15 @@ __global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
       }
     }
   }
+}
+
+template <class T, int RB, bool G>
+__global__ void __launch_bounds__(kTPB) k_fwd_step(DevPlan P, const int* wnode,
+                                                   const int* wtile, int s, double* x,
+                                                   int ldx, int nrhs, long long ustride,
+                                                   double* Y, long long ystride) {
+  fwd_step_wg<T, RB, G>(P, blockIdx.x, blockIdx.y, wnode, wtile, s, x, ldx, nrhs, ustride, Y,
+                        ystride);
 }
 
 // Backward step s: step q = (number of steps) - 1 - s, from the bottom.  Y
@@ -1717,14 +1730,13 @@ constexpr int kGemvCols = 16;
 // kPS pivots in Y (the first step gathered its own front vector), so they
 // are read there instead of from x.
 template <class T, int RB>
-__global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
-                                                   const int* wtile, const double* x,
-                                                   int ldx, int nrhs, double* Y,
-                                                   long long ystride, int ldxs, int yfw) {
-  extern __shared__ double xs[];  // x of the rows below the pivots, RB right-hand sides
-  const int p = wnode[blockIdx.x];
+__device__ __attribute__((always_inline)) inline void bwd_gemv_wg(
+    const DevPlan& P, int bx, int by, const int* wnode, const int* wtile, const double* x,
+    int ldx, int nrhs, double* Y, long long ystride, int ldxs, int yfw,
+    double* xs) {  // xs: LDS for x of the rows below the pivots, RB right-hand sides
+  const int p = wnode[bx];
   const int m = P.m[p], npiv = P.npiv[p];
-  const int j0 = kGemvCols * wtile[blockIdx.x];
+  const int j0 = kGemvCols * wtile[bx];
   if (j0 >= npiv) return;
   const int fs = P.ns[p] + P.din[p];
   const int cp = P.cptr[p];
@@ -1734,7 +1746,7 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   const int nr = m - npiv;
   const int yk = yfw ? min(npiv, kPS) : 0;
   constexpr int RR = RB > 0 ? RB : 1;
-  const int r0 = blockIdx.y * RR;
+  const int r0 = by * RR;
   const int nb = min(nrhs - r0, RR);
   // both index loads issued (clamped into their arrays; a node without
   // contribution rows reads vars twice), then the gather
@@ -1819,6 +1831,16 @@ __global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
   }
 }
 
+template <class T, int RB>
+__global__ void __launch_bounds__(kTPB) k_bwd_gemv(DevPlan P, const int* wnode,
+                                                   const int* wtile, const double* x,
+                                                   int ldx, int nrhs, double* Y,
+                                                   long long ystride, int ldxs, int yfw) {
+  extern __shared__ double xs[];
+  bwd_gemv_wg<T, RB>(P, blockIdx.x, blockIdx.y, wnode, wtile, x, ldx, nrhs, Y, ystride, ldxs,
+                     yfw, xs);
+}
+
 // Forward gather of a BIG node for RB right-hand sides per grid row: the
 // pivot rows of the front vector to Y, the rows below to U (the panel steps
 // solve them).  Each row's child lookups are done once for all the
@@ -1861,6 +1883,43 @@ __global__ void __launch_bounds__(1024) k_gather_big(DevPlan P, const int* nodes
       else U[r * ustride + (i - npiv)] = v[rr];
     }
   }
+}
+
+// One launch for a level's small fronts and the first forward step of its
+// large nodes, which gathers their front vectors itself (one right-hand
+// side): the two are independent (every child of either is at an earlier
+// level).  Workgroups 0 .. nsmall - 1 take the small fronts (LDS shape of P),
+// the rest the step's tiles.
+template <class T, bool REG>
+__global__ void __launch_bounds__(kTPB) k_fwd_level(DevPlan P, const int* snodes, int nsmall,
+                                                    double* x, int ldx, long long ustride,
+                                                    int chol, const int* wnode, const int* wtile,
+                                                    double* Y, long long ystride) {
+  if ((int)blockIdx.x < nsmall) {
+    extern __shared__ double t[];
+    __shared__ ChildView<T> s_cv[kChildCache];
+    __shared__ long long s_uo[kChildCache];
+    forward_node<T, kTPB, REG>(P, snodes[blockIdx.x], 0, x, ldx, ustride, chol, t, s_cv, s_uo);
+  } else {
+    fwd_step_wg<T, 1, true>(P, blockIdx.x - nsmall, 0, wnode, wtile, 0, x, ldx, 1, ustride, Y,
+                            ystride);
+  }
+}
+
+// The backward counterpart: a level's small fronts (their parents are at
+// later levels, done) and the L21^T products of its large nodes in one
+// launch (one right-hand side; the products stage the rows below in LDS).
+template <class T, bool REG>
+__global__ void __launch_bounds__(kTPB) k_bwd_level(DevPlan P, const int* snodes, int nsmall,
+                                                    double* x, int ldx, int chol,
+                                                    const int* bnode, const int* btile, double* Y,
+                                                    long long ystride, int ldxs, int yfw) {
+  extern __shared__ double lds[];
+  if ((int)blockIdx.x < nsmall)
+    backward_node<T, kTPB, REG>(P, snodes[blockIdx.x], 0, x, ldx, chol, lds);
+  else
+    bwd_gemv_wg<T, 1>(P, blockIdx.x - nsmall, 0, bnode, btile, x, ldx, 1, Y, ystride, ldxs, yfw,
+                      lds);
 }
 
 // r = b - (K + diag(d)) x for nrhs columns; 16 lanes per row.
@@ -2093,6 +2152,9 @@ struct Handle {
   // (same results, bit for bit)
   std::vector<char> lvl_gfuse;
   bool fuse_gather = true;
+  // KKT_LDL_MERGE_LEVELS=0 keeps a level's small fronts in launches of their
+  // own (k_fwd_level / k_bwd_level otherwise; same results, bit for bit)
+  bool merge_levels = true;
   DBuf<long long> d_pinvoff;
   DBuf<int> d_inv_node, d_inv_panel, d_inv2_node, d_inv2_pair;
   DBuf<double> d_pinv;
@@ -3136,6 +3198,13 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
   auto gfuse = [&](int l) {
     return H.fuse_gather && nrhs == 1 && (int)H.lvl_gfuse.size() > l && H.lvl_gfuse[l] != 0;
   };
+  // levels whose small fronts (one group: no tiny launch of their own) run
+  // in the launch of the large nodes' fused first step (forward) and L21^T
+  // products (backward)
+  auto merge_small = [&](int l, int s0, int s1, int sm, int g0, int g1) {
+    return H.merge_levels && gfuse(l) && g1 > g0 && sm == s0 && s1 > s0 &&
+           s1 - s0 <= kMergeSmall;
+  };
   // a chain's LDS shape and launch (one workgroup per right-hand side)
   auto chain_plan = [&](int l0, size_t* bytes) {
     DevPlan Q = P;
@@ -3159,10 +3228,28 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
     const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
     // a few tiny fronts join the level's other small fronts in one launch
     const int sm = s0 + (tiny_alone(l) ? H.sv_tiny[l] : 0);
+    const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
+    // few small fronts (all in one group) and a first step that gathers:
+    // both in one launch
+    const bool mrg = merge_small(l, s0, s1, sm, g0, g1);
+    if (mrg) {
+      const DevPlan Q = sv_plan(2 * l + 1, &lds, H.sv_tiny[l] > 0);
+      const int slot = H.fs_lvl[l];
+      const int f0 = H.fs_ptr[slot], f1 = H.fs_ptr[slot + 1];
+      const dim3 grid((unsigned)(s1 - s0 + f1 - f0));
+      if (s1 - s0 <= kRegLaunch)
+        hipLaunchKernelGGL((k_fwd_level<T, true>), grid, dim3(kTPB), lds, st, Q,
+                           H.d_sv_nodes.p + s0, s1 - s0, dx, ldx, us, chol, H.d_fs_node.p + f0,
+                           H.d_fs_tile.p + f0, Y, ys);
+      else
+        hipLaunchKernelGGL((k_fwd_level<T, false>), grid, dim3(kTPB), lds, st, Q,
+                           H.d_sv_nodes.p + s0, s1 - s0, dx, ldx, us, chol, H.d_fs_node.p + f0,
+                           H.d_fs_tile.p + f0, Y, ys);
+    }
     // launches of few workgroups are latency bound: their diagonal blocks
     // are read straight into registers (REG); larger ones stage them in LDS
     // and keep more workgroups per CU (same results either way)
-    if (sm > s0) {
+    if (!mrg && sm > s0) {
       const DevPlan Q = sv_plan(2 * l, &lds);
       if ((sm - s0) * nrhs <= kRegLaunch)
         hipLaunchKernelGGL((k_forward<T, kWave, true>), dim3(sm - s0, nrhs), dim3(kWave), lds, st,
@@ -3171,7 +3258,7 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
         hipLaunchKernelGGL((k_forward<T, kWave, false>), dim3(sm - s0, nrhs), dim3(kWave), lds, st,
                            Q, H.d_sv_nodes.p + s0, dx, ldx, nrhs, us, chol);
     }
-    if (s1 > sm) {
+    if (!mrg && s1 > sm) {
       const DevPlan Q = sv_plan(2 * l + 1, &lds, sm == s0 && H.sv_tiny[l] > 0);
       if ((s1 - sm) * nrhs <= kRegLaunch)
         hipLaunchKernelGGL((k_forward<T, kTPB, true>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
@@ -3180,7 +3267,6 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
         hipLaunchKernelGGL((k_forward<T, kTPB, false>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
                            Q, H.d_sv_nodes.p + sm, dx, ldx, nrhs, us, chol);
     }
-    const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
     if (g1 > g0) {
       // the gather done by the first step's workgroups where every large
       // node of the level has pivots (else a gather launch)
@@ -3196,7 +3282,7 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
                              us, Y, ys);
       }
       const int nst = std::min(H.fs_lvl[l + 1] - H.fs_lvl[l], H.lvl_steps[l]);
-      for (int t = 0; t < nst; t++) {
+      for (int t = mrg ? 1 : 0; t < nst; t++) {
         const int slot = H.fs_lvl[l] + t;
         const int f0 = H.fs_ptr[slot], f1 = H.fs_ptr[slot + 1];
         const dim3 g1r(f1 - f0, 1), g4r(f1 - f0, (nrhs + kStepRhs - 1) / kStepRhs);
@@ -3232,13 +3318,29 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
       continue;
     }
     const int g0 = H.large_ptr[l], g1 = H.large_ptr[l + 1];
+    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
+    const int sm = s0 + (tiny_alone(l) ? H.sv_tiny[l] : 0);
+    // few small fronts (one group) with the L21^T products: one launch
+    const bool mrg = merge_small(l, s0, s1, sm, g0, g1) && lds_bg <= kMergeLds;
     if (g1 > g0) {
       const int b0 = H.bg_ptr[l], b1 = H.bg_ptr[l + 1];
       const int yfw = gfuse(l) ? 1 : 0;
       // one right-hand side per workgroup (batching them here serialises a
       // workgroup's dot products and was measured slower); the x rows below
       // the pivots staged in LDS where they fit
-      if (lds_bg <= 128 * 1024)
+      if (mrg) {
+        const DevPlan Q = sv_plan(2 * l + 1, &lds, H.sv_tiny[l] > 0);
+        const dim3 grid((unsigned)(s1 - s0 + b1 - b0));
+        const size_t bytes = std::max(lds, lds_bg);
+        if (s1 - s0 <= kRegLaunch)
+          hipLaunchKernelGGL((k_bwd_level<T, true>), grid, dim3(kTPB), bytes, st, Q,
+                             H.d_sv_nodes.p + s0, s1 - s0, dx, ldx, chol, H.d_bg_node.p + b0,
+                             H.d_bg_tile.p + b0, Y, ys, H.max_below, yfw);
+        else
+          hipLaunchKernelGGL((k_bwd_level<T, false>), grid, dim3(kTPB), bytes, st, Q,
+                             H.d_sv_nodes.p + s0, s1 - s0, dx, ldx, chol, H.d_bg_node.p + b0,
+                             H.d_bg_tile.p + b0, Y, ys, H.max_below, yfw);
+      } else if (lds_bg <= 128 * 1024)
         hipLaunchKernelGGL((k_bwd_gemv<T, 1>), dim3(b1 - b0, nrhs), dim3(kTPB), lds_bg, st, P,
                            H.d_bg_node.p + b0, H.d_bg_tile.p + b0, dx, ldx, nrhs, Y, ys,
                            H.max_below, yfw);
@@ -3258,9 +3360,7 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
                              P, H.d_bs_node.p + f0, H.d_bs_tile.p + f0, t, dx, ldx, nrhs, Y, ys);
       }
     }
-    const int s0 = H.sv_ptr[l], s1 = H.sv_ptr[l + 1];
-    const int sm = s0 + (tiny_alone(l) ? H.sv_tiny[l] : 0);
-    if (sm > s0) {
+    if (!mrg && sm > s0) {
       const DevPlan Q = sv_plan(2 * l, &lds);
       if ((sm - s0) * nrhs <= kRegLaunch)
         hipLaunchKernelGGL((k_backward<T, kWave, true>), dim3(sm - s0, nrhs), dim3(kWave), lds, st,
@@ -3269,7 +3369,7 @@ static void run_solve_levels_t(Handle& H, double* dx, int ldx, int nrhs, hipStre
         hipLaunchKernelGGL((k_backward<T, kWave, false>), dim3(sm - s0, nrhs), dim3(kWave), lds, st,
                            Q, H.d_sv_nodes.p + s0, dx, ldx, nrhs, chol);
     }
-    if (s1 > sm) {
+    if (!mrg && s1 > sm) {
       const DevPlan Q = sv_plan(2 * l + 1, &lds, sm == s0 && H.sv_tiny[l] > 0);
       if ((s1 - sm) * nrhs <= kRegLaunch)
         hipLaunchKernelGGL((k_backward<T, kTPB, true>), dim3(s1 - sm, nrhs), dim3(kTPB), lds, st,
@@ -3459,6 +3559,7 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   H.lds_front = std::min(H.lds_front, rl_max_front());
   if (const char* e = std::getenv("KKT_LDL_ROOT_BK")) H.root_bk = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_FUSE_GATHER")) H.fuse_gather = std::atoi(e) != 0;
+  if (const char* e = std::getenv("KKT_LDL_MERGE_LEVELS")) H.merge_levels = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_SOLVE_CHAINS")) H.solve_chains = std::atoi(e) != 0;
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));

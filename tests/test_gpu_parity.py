@@ -224,15 +224,10 @@ def test_multi_rhs_columns_bitwise_equal_single_solves(prec):
         assert torch.equal(xj, X[:, j]), j
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("name", ["poisson2d", "cartpole"])
-def test_first_step_gather_bitwise_equal_gather_launch(name, prec):
-    """Levels whose large nodes all hold pivots gather the front vector in
-    the first forward step's workgroups (each the step's rows and its own
-    tile's rows, with k_gather_big's sums) instead of a gather launch before
-    the step (KKT_LDL_FUSE_GATHER=0 when the handle is created; solves of
-    one right-hand side): the same solutions bit for bit, for 1 and 5
-    right-hand sides and the refined solve, under nested dissection."""
+def _solves_bitwise_equal_under_switch(name, prec, var):
+    """Handles created with the environment switch `var` at 1 and at 0
+    give the same solutions bit for bit, for 1 and 5 right-hand sides and the
+    refined solve, under nested dissection."""
     import os
 
     import torch
@@ -240,13 +235,13 @@ def test_first_step_gather_bitwise_equal_gather_launch(name, prec):
     P = W.GENERATORS[name]()
     hs = []
     for flag in ("1", "0"):
-        os.environ["KKT_LDL_FUSE_GATHER"] = flag
+        os.environ[var] = flag
         try:
             s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
                           order=OrderingType.NESTED_DISSECTION,
                           precision=FP32_REFINE if prec == "fp32" else 0)
         finally:
-            del os.environ["KKT_LDL_FUSE_GATHER"]
+            del os.environ[var]
         assert s.factor(P.diag) == 0
         hs.append(s)
     g = torch.Generator(device="cuda").manual_seed(9)
@@ -262,3 +257,24 @@ def test_first_step_gather_bitwise_equal_gather_launch(name, prec):
     assert hs[1].solve(xb, refine=True) == 0
     assert torch.equal(xa, xb)
     assert _relres(P.matrix(), xa.cpu().numpy(), P.rhs) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["poisson2d", "cartpole"])
+def test_first_step_gather_bitwise_equal_gather_launch(name, prec):
+    """Levels whose large nodes all hold pivots gather the front vector in
+    the first forward step's workgroups (each the step's rows and its own
+    tile's rows, with k_gather_big's sums) instead of a gather launch before
+    the step (KKT_LDL_FUSE_GATHER=0 when the handle is created; solves of
+    one right-hand side): the same solutions bit for bit."""
+    _solves_bitwise_equal_under_switch(name, prec, "KKT_LDL_FUSE_GATHER")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["poisson2d", "cartpole"])
+def test_merged_level_launches_bitwise_equal(name, prec):
+    """A level's small fronts run in the launch of its large nodes' fused
+    first forward step and of their backward L21^T products (k_fwd_level,
+    k_bwd_level; one right-hand side) instead of launches of their own
+    (KKT_LDL_MERGE_LEVELS=0): the same solutions bit for bit."""
+    _solves_bitwise_equal_under_switch(name, prec, "KKT_LDL_MERGE_LEVELS")
