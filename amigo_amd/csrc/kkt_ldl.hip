@@ -260,29 +260,34 @@ __global__ void __launch_bounds__(kTPB) k_assemble(DevPlan P, const int* wnode,
       F[(size_t)j * m + j] = (T)((double)F[(size_t)j * m + j] + P.diag[P.node_to_var[vp + j]]);
 }
 
-// Assembly with the workgroup's kAsmCols columns staged in LDS (front
-// element type, leading dimension ldc >= m): the gather writes LDS only, so
+// Assembly with the workgroup's kAsmLdsCols columns, rows [r0, r0 +
+// kAsmRows), staged in LDS (front element type): the gather writes LDS only, so
 // no global store sits between one row's loads and the next row's (vector
 // memory operations complete in order), the original entries and Sigma are
 // added there with k_assemble's roundings, and the columns go to the front
 // in one pass.  The node is prepared here (the children's delay offsets by
 // a prefix sum over one wave; the workgroup of chunk 0 writes the node
 // state).  For nodes with at most kChildCache children and fronts of at
-// most kAsmLdsRows rows; bitwise equal to k_assemble.
+// most kAsmLdsRows rows; bitwise equal to k_assemble.  A work item is
+// (column chunk c, row chunk r), encoded c * kAsmRowChunks + r: the rows of
+// a wide front spread over several workgroups, so that its assembly costs
+// one round of map and entry loads per column instead of one per kTPB rows.
 constexpr int kAsmLdsRows = 2048;
 constexpr int kAsmLdsCols = 4;  // columns per workgroup of k_assemble_lds
+constexpr int kAsmRows = kTPB;  // rows per workgroup of k_assemble_lds
+constexpr int kAsmRowChunks = kAsmLdsRows / kAsmRows;
 template <class T>
 __global__ void __launch_bounds__(kTPB) k_assemble_lds(DevPlan P, const int* wnode,
-                                                       const int* wchunk, int ldc) {
-  extern __shared__ unsigned char s_raw[];
-  T* colv = (T*)s_raw;
+                                                       const int* wchunk) {
+  __shared__ T colv[kAsmLdsCols * kAsmRows];
   const int p = wnode[blockIdx.x];
   const int chunk = wchunk[blockIdx.x];
+  const int r0 = (chunk % kAsmRowChunks) * kAsmRows;
   const int tid = threadIdx.x, wv = tid / kWave, ln = tid % kWave;
   const int c0 = P.chptr[p], nch = P.chptr[p + 1] - c0;
   const int ns = P.ns[p], cs = P.cs[p];
   const int vp = P.vptr[p];
-  const int j0 = chunk * kAsmLdsCols;
+  const int j0 = (chunk / kAsmRowChunks) * kAsmLdsCols;
   __shared__ ChildView<T> s_cv[kChildCache];
   __shared__ int s_list[kAsmLdsCols][kChildCache];
   __shared__ int s_rj[kAsmLdsCols][kChildCache];
@@ -362,8 +367,9 @@ __global__ void __launch_bounds__(kTPB) k_assemble_lds(DevPlan P, const int* wno
   if (s_fail) return;
   const int din = s_din;
   const int m = ns + din + cs;
-  if (j0 >= m) return;
+  if (j0 >= m || r0 >= m) return;
   const int j1 = min(m, j0 + kAsmLdsCols);
+  const int r1 = min(m, r0 + kAsmRows);
   if (chunk == 0) {
     int* vars = P.vars + P.ioff[p];
     int* pre = P.pre + P.ioff[p];
@@ -394,8 +400,8 @@ __global__ void __launch_bounds__(kTPB) k_assemble_lds(DevPlan P, const int* wno
   for (int jj = 0; jj < j1 - j0; jj++) {
     const int j = j0 + jj;
     const int nl = s_nl[jj];
-    T* cj = colv + (size_t)jj * ldc;
-    for (int i = j + tid; i < m; i += kTPB) {
+    T* cj = colv + (size_t)jj * kAsmRows;
+    for (int i = max(j, r0) + tid; i < r1; i += kTPB) {
       double v = 0.0;
       // four children at a time: the map lookups, then the entries, all in
       // flight together; the sum stays in child order
@@ -418,30 +424,39 @@ __global__ void __launch_bounds__(kTPB) k_assemble_lds(DevPlan P, const int* wno
         for (int u = 0; u < 4; u++)
           if (ri[u] >= 0) v += (double)x[u];
       }
-      cj[i] = (T)v;
+      cj[i - r0] = (T)v;
     }
   }
   __syncthreads();
   // original entries (distinct positions), then Sigma, as k_assemble adds them
+  // (only those in the workgroup's rows)
   if (oe_u >= 0) {
-    T* c = colv + (size_t)(oe_j - j0) * ldc + (oe_u < ns ? oe_u : oe_u + din);
-    *c = (T)((double)*c + oe_v);
+    const int i = oe_u < ns ? oe_u : oe_u + din;
+    if (i >= r0 && i < r1) {
+      T* c = colv + (size_t)(oe_j - j0) * kAsmRows + (i - r0);
+      *c = (T)((double)*c + oe_v);
+    }
   }
   for (int e = e0 + kTPB + tid; e < e1; e += kTPB) {
     const int u = P.am_row[e], j = P.am_col[e];
-    T* c = colv + (size_t)(j - j0) * ldc + (u < ns ? u : u + din);
-    *c = (T)((double)*c + P.vals[P.am_src[e]]);
+    const int i = u < ns ? u : u + din;
+    if (i >= r0 && i < r1) {
+      T* c = colv + (size_t)(j - j0) * kAsmRows + (i - r0);
+      *c = (T)((double)*c + P.vals[P.am_src[e]]);
+    }
   }
   __syncthreads();
-  if (P.diag && tid < kAsmLdsCols && j0 + tid < min(j1, ns)) {
-    T* c = colv + (size_t)tid * ldc + j0 + tid;
+  if (P.diag && tid < kAsmLdsCols && j0 + tid < min(j1, ns) && j0 + tid >= r0 &&
+      j0 + tid < r1) {
+    T* c = colv + (size_t)tid * kAsmRows + (j0 + tid - r0);
     *c = (T)((double)*c + dg);
   }
   __syncthreads();
   T* F = front_ptr<T>(P, p);
   for (int jj = 0; jj < j1 - j0; jj++) {
     const int j = j0 + jj;
-    for (int i = j + tid; i < m; i += kTPB) F[(size_t)j * m + i] = colv[(size_t)jj * ldc + i];
+    for (int i = max(j, r0) + tid; i < r1; i += kTPB)
+      F[(size_t)j * m + i] = colv[(size_t)jj * kAsmRows + (i - r0)];
   }
 }
 
@@ -2360,9 +2375,15 @@ static void build_plan(Handle& H, const std::vector<int>& slack) {
       const int k = H.tk_grp_nodes[q];
       if (H.cap[k] <= H.lds_front) continue;  // assembled inside the LDS front kernels
       const int nch = (H.cap[k] + width - 1) / width;
+      // k_assemble_lds: the row chunks of each column chunk from the one
+      // holding its first column (rows above a column are not assembled)
+      const int nrc = H.tk_asm_lds[t] ? (H.cap[k] + kAsmRows - 1) / kAsmRows : 1;
       for (int c = 0; c < nch; c++) {
-        H.asm_node.push_back(k);
-        H.asm_chunk.push_back(c);
+        const int rc0 = H.tk_asm_lds[t] ? c * width / kAsmRows : 0;
+        for (int r = rc0; r < nrc; r++) {
+          H.asm_node.push_back(k);
+          H.asm_chunk.push_back(H.tk_asm_lds[t] ? c * kAsmRowChunks + r : c);
+        }
       }
     }
     H.tk_asm_ptr[t + 1] = (int)H.asm_node.size();
@@ -2961,11 +2982,9 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       }
       if (a1 > a0) {
         ProfScope ps_(H, st, KC_ASSEMBLE);
-        const int ldc = H.tk_asm_cap[t];
         if (H.tk_asm_lds[t])
-          hipLaunchKernelGGL(k_assemble_lds<T>, dim3(a1 - a0), dim3(kTPB),
-                             sizeof(T) * kAsmLdsCols * (size_t)ldc, st, P, H.d_asm_node.p + a0,
-                             H.d_asm_chunk.p + a0, ldc);
+          hipLaunchKernelGGL(k_assemble_lds<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
+                             H.d_asm_node.p + a0, H.d_asm_chunk.p + a0);
         else
           hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(kTPB), 0, st, P,
                              H.d_asm_node.p + a0, H.d_asm_chunk.p + a0, H.tk_prep[t] ? 0 : 1);
