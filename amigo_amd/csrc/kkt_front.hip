@@ -32,9 +32,20 @@ namespace kktldl {
 
 constexpr int kRLThreads = 256;
 constexpr int kRLMaxFront = 96;  // largest front dimension of these kernels
-// trailing-triangle entries per thread: ceil(96 * 97 / 2 / 256)
-constexpr int kRLEntries = (kRLMaxFront * (kRLMaxFront + 1) / 2 + kRLThreads - 1) / kRLThreads;
-constexpr int kRLBatch = 10;  // trailing entries whose loads are issued together
+// trailing-triangle entries per thread for fronts of at most MF rows:
+// ceil(MF (MF + 1) / 2 / 256)
+template <int MF>
+constexpr int rl_entries() {
+  return (MF * (MF + 1) / 2 + kRLThreads - 1) / kRLThreads;
+}
+// trailing entries whose loads are issued together
+template <int MF>
+constexpr int rl_batch() {
+  return rl_entries<MF>() < 10 ? rl_entries<MF>() : 10;
+}
+// launches whose fronts have at most 32, 48 or 64 rows take instances with
+// fewer entries per thread (fewer registers: more workgroups per CU; the
+// same arithmetic)
 
 template <class T>
 __device__ inline double round_t(double x) {
@@ -46,11 +57,13 @@ __device__ inline double round_t(double x) {
 // of the front (row m - 1 - ir, column m - 1 - jr): the entries of the
 // trailing triangle from column j0 on are the prefix e < (m - j0)(m - j0 +
 // 1) / 2 whatever m, so one table serves every front and step.
+template <int MF>
 struct TriEntries {
-  int pk[kRLEntries];
+  static constexpr int kN = rl_entries<MF>();
+  int pk[kN];
   __device__ void init(int tid) {
 #pragma unroll
-    for (int r = 0; r < kRLEntries; r++) {
+    for (int r = 0; r < kN; r++) {
       const int e = tid + kRLThreads * r;
       int jr = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
       while (jr * (jr + 1) / 2 > e) jr--;
@@ -101,9 +114,11 @@ __device__ inline bool rl_prepare(const DevPlan& P, int p, RLShared& sh) {
 
 // One front: assemble into LDS (FP64), factor, write back.  lds: F (m x m)
 // | dv | dv2 | vars | pre | ptype.
-template <class T>
-__device__ void front_rl(const DevPlan& P, int p, double* lds, const TriEntries& te,
+template <class T, int MF>
+__device__ void front_rl(const DevPlan& P, int p, double* lds, const TriEntries<MF>& te,
                          RLShared& sh) {
+  constexpr int kRLEntries = rl_entries<MF>();
+  constexpr int kRLBatch = rl_batch<MF>();
   const int tid = threadIdx.x, lane = tid % kWave;
   if (!rl_prepare(P, p, sh)) return;
   const int din = sh.din;
@@ -420,13 +435,13 @@ __device__ void front_rl(const DevPlan& P, int p, double* lds, const TriEntries&
   }
 }
 
-template <class T>
+template <class T, int MF>
 __global__ void __launch_bounds__(kRLThreads) k_front_rl(DevPlan P, const int* nodes) {
   extern __shared__ double lds[];
   __shared__ RLShared sh;
-  TriEntries te;
+  TriEntries<MF> te;
   te.init(threadIdx.x);
-  front_rl<T>(P, nodes[blockIdx.x], lds, te, sh);
+  front_rl<T, MF>(P, nodes[blockIdx.x], lds, te, sh);
 }
 
 // The fronts of `nodes` one after another (each front's children are
@@ -439,10 +454,10 @@ __global__ void __launch_bounds__(kRLThreads) k_front_chain(DevPlan P, const int
                                                             int count) {
   extern __shared__ double lds[];
   __shared__ RLShared sh;
-  TriEntries te;
+  TriEntries<kRLMaxFront> te;
   te.init(threadIdx.x);
   for (int q = 0; q < count; q++) {
-    front_rl<T>(P, nodes[q], lds, te, sh);
+    front_rl<T, kRLMaxFront>(P, nodes[q], lds, te, sh);
     __syncthreads();
   }
 }
@@ -460,8 +475,15 @@ static void launch_rl(int cap, int count, bool chain, hipStream_t st, const DevP
   const size_t bytes = rl_front_bytes(cap);
   if (chain)
     hipLaunchKernelGGL(k_front_chain<T>, dim3(1), dim3(kRLThreads), bytes, st, P, nodes, count);
+  else if (cap <= 32)
+    hipLaunchKernelGGL((k_front_rl<T, 32>), dim3(count), dim3(kRLThreads), bytes, st, P, nodes);
+  else if (cap <= 48)
+    hipLaunchKernelGGL((k_front_rl<T, 48>), dim3(count), dim3(kRLThreads), bytes, st, P, nodes);
+  else if (cap <= 64)
+    hipLaunchKernelGGL((k_front_rl<T, 64>), dim3(count), dim3(kRLThreads), bytes, st, P, nodes);
   else
-    hipLaunchKernelGGL(k_front_rl<T>, dim3(count), dim3(kRLThreads), bytes, st, P, nodes);
+    hipLaunchKernelGGL((k_front_rl<T, kRLMaxFront>), dim3(count), dim3(kRLThreads), bytes, st, P,
+                       nodes);
 }
 
 void launch_front_rl(const float*, int cap, int count, bool chain, hipStream_t st,
