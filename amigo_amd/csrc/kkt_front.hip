@@ -38,14 +38,16 @@ template <int MF>
 constexpr int rl_entries() {
   return (MF * (MF + 1) / 2 + kRLThreads - 1) / kRLThreads;
 }
-// trailing entries whose loads are issued together
+// trailing entries whose loads are issued together: at most 10, in as few
+// rounds as that allows, split evenly
 template <int MF>
 constexpr int rl_batch() {
-  return rl_entries<MF>() < 10 ? rl_entries<MF>() : 10;
+  constexpr int n = rl_entries<MF>(), rounds = (n + 9) / 10;
+  return (n + rounds - 1) / rounds;
 }
-// launches whose fronts have at most 32, 48 or 64 rows take instances with
-// fewer entries per thread (fewer registers: more workgroups per CU; the
-// same arithmetic)
+// launches whose fronts have at most 32, 48, 64 or 80 rows take instances
+// with fewer entries per thread (fewer registers: more workgroups per CU;
+// the same arithmetic)
 
 template <class T>
 __device__ inline double round_t(double x) {
@@ -481,6 +483,8 @@ static void launch_rl(int cap, int count, bool chain, hipStream_t st, const DevP
     hipLaunchKernelGGL((k_front_rl<T, 48>), dim3(count), dim3(kRLThreads), bytes, st, P, nodes);
   else if (cap <= 64)
     hipLaunchKernelGGL((k_front_rl<T, 64>), dim3(count), dim3(kRLThreads), bytes, st, P, nodes);
+  else if (cap <= 80)
+    hipLaunchKernelGGL((k_front_rl<T, 80>), dim3(count), dim3(kRLThreads), bytes, st, P, nodes);
   else
     hipLaunchKernelGGL((k_front_rl<T, kRLMaxFront>), dim3(count), dim3(kRLThreads), bytes, st, P,
                        nodes);
