@@ -694,10 +694,12 @@ bool launch_large_panel(const double*, bool bkc, int maxcap, dim3 grid, hipStrea
 
 // Fronts held whole in LDS (kkt_front.hip), threshold / Bunch-Kaufman modes:
 // `count` fronts of at most `cap` rows, one workgroup each (chain = false),
-// or one workgroup factoring them in list order (chain = true).
-void launch_front_rl(const float*, int cap, int count, bool chain, hipStream_t st,
+// or one workgroup factoring them in list order (chain = true).  sized:
+// the kernel instance for the launch's front bound (else the one for the
+// largest fronts; the same results).
+void launch_front_rl(const float*, int cap, int count, bool chain, bool sized, hipStream_t st,
                      const DevPlan& P, const int* nodes);
-void launch_front_rl(const double*, int cap, int count, bool chain, hipStream_t st,
+void launch_front_rl(const double*, int cap, int count, bool chain, bool sized, hipStream_t st,
                      const DevPlan& P, const int* nodes);
 int rl_max_front();
 

@@ -119,6 +119,10 @@ __device__ inline bool rl_prepare(const DevPlan& P, int p, RLShared& sh) {
 template <class T, int MF>
 __device__ void front_rl(const DevPlan& P, int p, double* lds, const TriEntries<MF>& te,
                          RLShared& sh) {
+  // every product rounded before it is added (no fused multiply-adds): the
+  // compiler would otherwise fuse them per instance, and the instances for
+  // different front bounds would differ in the last bits
+#pragma clang fp contract(off)
   constexpr int kRLEntries = rl_entries<MF>();
   constexpr int kRLBatch = rl_batch<MF>();
   const int tid = threadIdx.x, lane = tid % kWave;
@@ -471,12 +475,15 @@ size_t rl_front_bytes(int c) {
 int rl_max_front() { return kRLMaxFront; }
 
 template <class T>
-static void launch_rl(int cap, int count, bool chain, hipStream_t st, const DevPlan& P,
-                      const int* nodes) {
+static void launch_rl(int cap, int count, bool chain, bool sized, hipStream_t st,
+                      const DevPlan& P, const int* nodes) {
   if (cap > kRLMaxFront) throw Error(-2, "LDS front above the right-looking kernel's bound");
   const size_t bytes = rl_front_bytes(cap);
   if (chain)
     hipLaunchKernelGGL(k_front_chain<T>, dim3(1), dim3(kRLThreads), bytes, st, P, nodes, count);
+  else if (!sized)
+    hipLaunchKernelGGL((k_front_rl<T, kRLMaxFront>), dim3(count), dim3(kRLThreads), bytes, st, P,
+                       nodes);
   else if (cap <= 32)
     hipLaunchKernelGGL((k_front_rl<T, 32>), dim3(count), dim3(kRLThreads), bytes, st, P, nodes);
   else if (cap <= 48)
@@ -490,13 +497,13 @@ static void launch_rl(int cap, int count, bool chain, hipStream_t st, const DevP
                        nodes);
 }
 
-void launch_front_rl(const float*, int cap, int count, bool chain, hipStream_t st,
+void launch_front_rl(const float*, int cap, int count, bool chain, bool sized, hipStream_t st,
                      const DevPlan& P, const int* nodes) {
-  launch_rl<float>(cap, count, chain, st, P, nodes);
+  launch_rl<float>(cap, count, chain, sized, st, P, nodes);
 }
-void launch_front_rl(const double*, int cap, int count, bool chain, hipStream_t st,
+void launch_front_rl(const double*, int cap, int count, bool chain, bool sized, hipStream_t st,
                      const DevPlan& P, const int* nodes) {
-  launch_rl<double>(cap, count, chain, st, P, nodes);
+  launch_rl<double>(cap, count, chain, sized, st, P, nodes);
 }
 
 }  // namespace kktldl

@@ -2170,6 +2170,10 @@ struct Handle {
   // KKT_LDL_MERGE_LEVELS=0 keeps a level's small fronts in launches of their
   // own (k_fwd_level / k_bwd_level otherwise; same results, bit for bit)
   bool merge_levels = true;
+  // KKT_LDL_RL_INSTANCES=0: every LDS front launch takes the kernel instance
+  // for the largest fronts (else the one for the launch's front bound; the
+  // same results, bit for bit)
+  bool rl_sized = true;
   DBuf<long long> d_pinvoff;
   DBuf<int> d_inv_node, d_inv_panel, d_inv2_node, d_inv2_pair;
   DBuf<double> d_pinv;
@@ -2183,7 +2187,11 @@ struct Handle {
   bool uploaded = false;
   int u_nrhs = 0;
   kkt_ldl_stats stats;
-  std::vector<int> h_npiv, h_ndel, h_cnt, h_stat;
+  // host copies of the factorization's per-node results, in one pinned
+  // buffer (stat | npiv | ndel | cnt: 8 N ints) so that they arrive with one
+  // synchronisation
+  int* h_pin = nullptr;
+  int *h_stat = nullptr, *h_npiv = nullptr, *h_ndel = nullptr, *h_cnt = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   float last_factor_ms = 0.f, last_solve_ms = 0.f;
   // second stream (default priority) for the LDS fronts, which run beside
@@ -2927,7 +2935,7 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       }
       {
         ProfScope ps_(H, st, KC_FRONT_LDS);
-        launch_front_rl((const T*)nullptr, cap, e - t + 1, true, st, P,
+        launch_front_rl((const T*)nullptr, cap, e - t + 1, true, true, st, P,
                         H.d_tk_grp_nodes.p + H.tk_grp_ptr[t]);
       }
       t = e;
@@ -2969,7 +2977,8 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
           while (q1 < d1 && H.cap[H.tk_lds_nodes[q1]] <= bucket) q1++;
           ProfScope ps_(H, ss, KC_FRONT_LDS);
           if (H.rl_fronts) {
-            launch_front_rl((const T*)nullptr, std::min(bucket, H.lds_front), q1 - q0, false, ss,
+            launch_front_rl((const T*)nullptr, std::min(bucket, H.lds_front), q1 - q0, false,
+                            H.rl_sized, ss,
                             P, H.d_tk_lds_nodes.p + q0);
           } else {
             const size_t bytes = lds_front_bytes(bucket, sizeof(T));
@@ -3055,6 +3064,11 @@ static void ensure_device(Handle& H) {
   HIPCHECK(hipEventCreate(&H.ev3));
   HIPCHECK(hipStreamCreateWithFlags(&H.side, hipStreamNonBlocking));
   HIPCHECK(hipEventCreateWithFlags(&H.ev_fork, hipEventDisableTiming));
+  HIPCHECK(hipHostMalloc((void**)&H.h_pin, sizeof(int) * 8 * (size_t)std::max(H.N, 1)));
+  H.h_stat = H.h_pin;
+  H.h_npiv = H.h_pin + H.N;
+  H.h_ndel = H.h_pin + 2 * (size_t)H.N;
+  H.h_cnt = H.h_pin + 3 * (size_t)H.N;
   H.uploaded = true;
 }
 
@@ -3094,9 +3108,19 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
     HIPCHECK(hipEventRecord(H.ev0, st));
     run_factor_levels(H, st);
     HIPCHECK(hipEventRecord(H.ev1, st));
-    H.h_stat.resize(H.N);
-    HIPCHECK(hipMemcpyAsync(H.h_stat.data(), H.d_stat.p, sizeof(int) * H.N,
-                            hipMemcpyDeviceToHost, st));
+    // zero and non-finite pivots (nodes that failed count none), then the
+    // per-node results to the host, all behind one synchronisation
+    {
+      DevPlan P = dev_plan(H);
+      hipLaunchKernelGGL(k_pivot_check, dim3((unsigned)H.N), dim3(kWave), 0, st, P,
+                         H.d_cnt.p + 3 * (size_t)H.N);
+      HIPCHECK(hipGetLastError());
+    }
+    const size_t nb = sizeof(int) * (size_t)H.N;
+    HIPCHECK(hipMemcpyAsync(H.h_stat, H.d_stat.p, nb, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(H.h_npiv, H.d_npiv.p, nb, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(H.h_ndel, H.d_ndel.p, nb, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(H.h_cnt, H.d_cnt.p, 5 * nb, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     bool fail = false;
     for (int k = 0; k < H.N; k++) fail |= H.h_stat[k] != 0;
@@ -3125,22 +3149,8 @@ static int factor_impl(Handle& H, const double* vals, const double* diag, int me
     H.d_Y.alloc((size_t)H.i_total);
     H.y_nrhs = 1;
   }
-  // zero and non-finite pivots of the (successful) factorization
-  {
-    DevPlan P = dev_plan(H);
-    hipLaunchKernelGGL(k_pivot_check, dim3((unsigned)H.N), dim3(kWave), 0, st, P,
-                       H.d_cnt.p + 3 * (size_t)H.N);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(st));
-  }
   HIPCHECK(hipEventElapsedTime(&H.last_factor_ms, H.ev0, H.ev1));
   prof_collect(H);
-  H.h_npiv.resize(H.N);
-  H.h_ndel.resize(H.N);
-  H.h_cnt.resize(5 * (size_t)H.N);
-  HIPCHECK(hipMemcpy(H.h_npiv.data(), H.d_npiv.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(H.h_ndel.data(), H.d_ndel.p, sizeof(int) * H.N, hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(H.h_cnt.data(), H.d_cnt.p, sizeof(int) * 5 * H.N, hipMemcpyDeviceToHost));
   // rows below the pivots of the large fronts (the backward GEMV's x staging)
   H.max_below = 0;
   for (int k : H.large_nodes) H.max_below = std::max(H.max_below, H.cap[k] - H.h_npiv[k]);
@@ -3579,6 +3589,7 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   if (const char* e = std::getenv("KKT_LDL_ROOT_BK")) H.root_bk = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_FUSE_GATHER")) H.fuse_gather = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_MERGE_LEVELS")) H.merge_levels = std::atoi(e) != 0;
+  if (const char* e = std::getenv("KKT_LDL_RL_INSTANCES")) H.rl_sized = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_SOLVE_CHAINS")) H.solve_chains = std::atoi(e) != 0;
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));
@@ -3914,6 +3925,7 @@ void kkt_ldl_destroy(kkt_ldl_t h) {
   if (H.ev_fork) (void)hipEventDestroy(H.ev_fork);
   for (auto e : H.lds_ev) (void)hipEventDestroy(e);
   if (H.side) (void)hipStreamDestroy(H.side);
+  if (H.h_pin) (void)hipHostFree(H.h_pin);
   delete h;
 }
 

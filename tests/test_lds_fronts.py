@@ -199,3 +199,39 @@ def test_chain_solve_bitwise_equal_level_launches(prec):
     assert hs[1].solve(xb, refine=True) == 0
     assert torch.equal(xa, xb)
     assert _relres(P.matrix(), xa.cpu().numpy(), P.rhs) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["poisson2d", "cartpole"])
+def test_lds_front_instances_bitwise_equal(name, prec):
+    """LDS front launches take the kernel instance for their front bound (32,
+    48, 64, 80 or 96 rows: fewer registers for smaller bounds); with
+    KKT_LDL_RL_INSTANCES=0 every launch takes the 96-row instance.  Products
+    are rounded before they are added in every instance, so the factors are
+    the same bit for bit: equal solutions for 1 right-hand side and for the
+    refined solve, equal inertia, under nested dissection."""
+    import torch
+    P = W.GENERATORS[name]()
+    hs = []
+    for flag in ("1", "0"):
+        os.environ["KKT_LDL_RL_INSTANCES"] = flag
+        try:
+            s = SparseLDL(CSRMat(P.n, P.n, P.rowp, P.cols, P.vals),
+                          order=OrderingType.NESTED_DISSECTION, precision=prec)
+        finally:
+            del os.environ["KKT_LDL_RL_INSTANCES"]
+        assert s.factor(P.diag) == 0
+        hs.append(s)
+    assert hs[0].get_inertia() == hs[1].get_inertia()
+    g = torch.Generator(device="cuda").manual_seed(5)
+    b = torch.randn(P.n, dtype=torch.float64, device="cuda", generator=g)
+    xa, xb = b.clone(), b.clone()
+    hs[0].solve(xa)
+    hs[1].solve(xb)
+    assert torch.equal(xa, xb)
+    xa = torch.as_tensor(P.rhs, device="cuda").clone()
+    xb = xa.clone()
+    assert hs[0].solve(xa, refine=True) == 0
+    assert hs[1].solve(xb, refine=True) == 0
+    assert torch.equal(xa, xb)
+    assert _relres(P.matrix(), xa.cpu().numpy(), P.rhs) <= 1e-10
