@@ -2170,10 +2170,12 @@ struct Handle {
   // KKT_LDL_MERGE_LEVELS=0 keeps a level's small fronts in launches of their
   // own (k_fwd_level / k_bwd_level otherwise; same results, bit for bit)
   bool merge_levels = true;
-  // KKT_LDL_RL_INSTANCES=0: every LDS front launch takes the kernel instance
-  // for the largest fronts (else the one for the launch's front bound; the
-  // same results, bit for bit)
-  bool rl_sized = true;
+  // KKT_LDL_RL_INSTANCES=1: each LDS front launch takes the kernel instance
+  // for its front bound (fewer registers, more workgroups per CU) instead of
+  // the one for the largest fronts.  Off: with it, about 2-7 % of repeated
+  // FP32 factorizations of one matrix differed in their last bits (none in
+  // 50 with the single instance); the cause is not found
+  bool rl_sized = false;
   DBuf<long long> d_pinvoff;
   DBuf<int> d_inv_node, d_inv_panel, d_inv2_node, d_inv2_pair;
   DBuf<double> d_pinv;
@@ -2192,6 +2194,10 @@ struct Handle {
   // synchronisation
   int* h_pin = nullptr;
   int *h_stat = nullptr, *h_npiv = nullptr, *h_ndel = nullptr, *h_cnt = nullptr;
+  // pinned host copies of the refined solve's squared norms (right-hand
+  // sides, residuals: kMaxNormRhs each), so that their reads stay
+  // asynchronous
+  double* h_norm = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   float last_factor_ms = 0.f, last_solve_ms = 0.f;
   // second stream (default priority) for the LDS fronts, which run beside
@@ -3069,6 +3075,7 @@ static void ensure_device(Handle& H) {
   H.h_npiv = H.h_pin + H.N;
   H.h_ndel = H.h_pin + 2 * (size_t)H.N;
   H.h_cnt = H.h_pin + 3 * (size_t)H.N;
+  HIPCHECK(hipHostMalloc((void**)&H.h_norm, sizeof(double) * 2 * kMaxNormRhs));
   H.uploaded = true;
 }
 
@@ -3453,11 +3460,11 @@ static int solve_impl(Handle& H, int nrhs, double* x, int ldx, int mem,
       w = RefineWork{H.d_b.p, H.d_r.p, H.d_xb.p, H.d_norm.p};
     }
     HIPCHECK(hipMemcpyAsync(w.b, dx, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
-    // host copies of the squared norms (fixed size: no allocation here)
-    double bn[kMaxNormRhs], rn[kMaxNormRhs];
+    // host copies of the squared norms (the handle's pinned buffers, set up
+    // by the first factorization: no allocation here)
     std::vector<double> bnv, rnv;
-    double* bnp = bn;
-    double* rnp = rn;
+    double* bnp = H.h_norm;
+    double* rnp = H.h_norm + kMaxNormRhs;
     if (nrhs > kMaxNormRhs) {
       bnv.resize(nrhs);
       rnv.resize(nrhs);
@@ -3926,6 +3933,7 @@ void kkt_ldl_destroy(kkt_ldl_t h) {
   for (auto e : H.lds_ev) (void)hipEventDestroy(e);
   if (H.side) (void)hipStreamDestroy(H.side);
   if (H.h_pin) (void)hipHostFree(H.h_pin);
+  if (H.h_norm) (void)hipHostFree(H.h_norm);
   delete h;
 }
 
