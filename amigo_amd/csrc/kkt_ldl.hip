@@ -2170,12 +2170,16 @@ struct Handle {
   // KKT_LDL_MERGE_LEVELS=0 keeps a level's small fronts in launches of their
   // own (k_fwd_level / k_bwd_level otherwise; same results, bit for bit)
   bool merge_levels = true;
-  // KKT_LDL_RL_INSTANCES=1: each LDS front launch takes the kernel instance
-  // for its front bound (fewer registers, more workgroups per CU) instead of
-  // the one for the largest fronts.  Off: with it, about 2-7 % of repeated
-  // FP32 factorizations of one matrix differed in their last bits (none in
-  // 50 with the single instance); the cause is not found
-  bool rl_sized = false;
+  // KKT_LDL_RL_INSTANCES=0: every LDS front launch takes the kernel
+  // instance for the largest fronts (else the one for its front bound:
+  // fewer registers, more workgroups per CU; the same results)
+  bool rl_sized = true;
+  // KKT_LDL_SIDE_STREAM=1: the LDS fronts of a tick that has other work run
+  // on the handle's second stream beside it.  Off by default: with it, some
+  // repeated FP32 factorizations of one matrix differed in their last bits
+  // (an ordering hazard between the two streams whose cause is not found;
+  // none in 180 factorizations on one stream)
+  bool side_lds = false;
   DBuf<long long> d_pinvoff;
   DBuf<int> d_inv_node, d_inv_panel, d_inv2_node, d_inv2_pair;
   DBuf<double> d_pinv;
@@ -2969,7 +2973,7 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
       // work, in size buckets, so that small fronts get small LDS
       // allocations and several workgroups per CU.
       if (d1 > d0) {
-        const int ev = H.tk_lds_ev[t];
+        const int ev = H.side_lds ? H.tk_lds_ev[t] : -1;
         hipStream_t ss = ev >= 0 ? H.side : st;
         if (ev >= 0) {
           HIPCHECK(hipEventRecord(H.ev_fork, st));
@@ -3597,6 +3601,7 @@ static void create_impl(Handle& H, int n, const int* rowp, const int* cols,
   if (const char* e = std::getenv("KKT_LDL_FUSE_GATHER")) H.fuse_gather = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_MERGE_LEVELS")) H.merge_levels = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_RL_INSTANCES")) H.rl_sized = std::atoi(e) != 0;
+  if (const char* e = std::getenv("KKT_LDL_SIDE_STREAM")) H.side_lds = std::atoi(e) != 0;
   if (const char* e = std::getenv("KKT_LDL_SOLVE_CHAINS")) H.solve_chains = std::atoi(e) != 0;
   build_plan(H, initial_slack(H));
   std::memset(&H.stats, 0, sizeof(H.stats));

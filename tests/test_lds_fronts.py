@@ -201,9 +201,6 @@ def test_chain_solve_bitwise_equal_level_launches(prec):
     assert _relres(P.matrix(), xa.cpu().numpy(), P.rhs) <= 1e-10
 
 
-@pytest.mark.skip(reason="the size instances of the LDS front kernel (KKT_LDL_RL_INSTANCES=1, "
-                         "off by default) gave run-to-run different FP32 factors in 2-7 % of "
-                         "repeated factorizations; cause not found")
 @pytest.mark.parametrize("prec", [FP64, FP32_REFINE], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("name", ["poisson2d", "cartpole"])
 def test_lds_front_instances_bitwise_equal(name, prec):
