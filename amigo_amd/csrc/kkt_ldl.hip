@@ -2204,9 +2204,10 @@ struct Handle {
   double* h_norm = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   float last_factor_ms = 0.f, last_solve_ms = 0.f;
-  // second stream (default priority) for the LDS fronts, which run beside
-  // the panel chain: ev_fork orders them after their k_prepare, lds_ev[]
-  // (one per tick with LDS fronts) orders their parents after them
+  // second stream (default priority) for the LDS fronts when side_lds is
+  // set, which run beside the panel chain: ev_fork orders them after their
+  // k_prepare, lds_ev[] (one per tick with LDS fronts) orders their parents
+  // after them
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr;
 };
@@ -2969,9 +2970,9 @@ static void run_factor_levels_t(Handle& H, hipStream_t st) {
                            P, H.d_tk_grp_nodes.p + n0, n1 - n0);
       }
       // The LDS fronts (assembled in LDS) depend only on k_prepare and on
-      // their children; they run on the side stream beside the tick's other
-      // work, in size buckets, so that small fronts get small LDS
-      // allocations and several workgroups per CU.
+      // their children; with side_lds they run on the side stream beside the
+      // tick's other work (else on this stream), in size buckets, so that
+      // small fronts get small LDS allocations and several workgroups per CU.
       if (d1 > d0) {
         const int ev = H.side_lds ? H.tk_lds_ev[t] : -1;
         hipStream_t ss = ev >= 0 ? H.side : st;
